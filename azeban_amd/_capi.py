@@ -1,0 +1,82 @@
+"""ctypes binding of the C ABI in include/azeban_hip.h (libazeban_hip.so).
+
+Loading fails loudly when the library has not been built: there is no CPU
+fallback for any entry point.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libazeban_hip.so"
+
+AZB_FP64 = 0
+AZB_FP32_MIXED = 1
+
+
+class AzbError(RuntimeError):
+    pass
+
+
+class SVParams(C.Structure):
+    _fields_ = [("eps", C.c_double), ("s", C.c_double), ("k0", C.c_double)]
+
+
+_vp = C.c_void_p
+_i = C.c_int
+_d = C.c_double
+_l = C.c_long
+_sz = C.c_size_t
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+# name -> (restype, argtypes); the full list of exported entry points
+SIGNATURES = {
+    "azb_last_error": (C.c_char_p, []),
+    "azb_create": (_i, [_i, _i, _i, _i, _i, SVParams, _vp, C.POINTER(_vp)]),
+    "azb_destroy": (_i, [_vp]),
+    "azb_workspace_bytes": (_sz, [_vp]),
+    "azb_stream": (_vp, [_vp]),
+    "azb_dudt": (_i, [_vp, _vp, _vp]),
+    "azb_dt": (_i, [_vp, _d, _dp]),
+    "azb_u_max": (_i, [_vp, _dp]),
+    "azb_rk3_step": (_i, [_vp, _vp, _d, _d, _dp]),
+    "azb_simulate_until": (_i, [_vp, _vp, _d, _d, _dp, _i, _i, _ip]),
+    "azb_fft_r2c": (_i, [_i, _i, _i, _i, _vp, _vp, _vp]),
+    "azb_fft_c2r": (_i, [_i, _i, _i, _i, _vp, _vp, _vp]),
+    "azb_copy_to_padded": (_i, [_i, _i, _i, _vp, _vp, _vp]),
+    "azb_axpby": (_i, [_d, _vp, _d, _vp, _l, _vp]),
+    "azb_max_norm": (_i, [_vp, _l, _dp, _vp]),
+    "azb_leray": (_i, [_i, _i, _vp, _vp]),
+    "azb_energy_spectrum": (_i, [_i, _i, _vp, _dp, _vp]),
+    "azb_malloc": (_i, [C.POINTER(_vp), _sz]),
+    "azb_free": (_i, [_vp]),
+    "azb_memcpy_h2d": (_i, [_vp, _vp, _sz]),
+    "azb_memcpy_d2h": (_i, [_vp, _vp, _sz]),
+    "azb_synchronize": (_i, [_vp]),
+    "azb_device_synchronize": (_i, []),
+}
+
+_lib = None
+
+
+def load() -> C.CDLL:
+    """Load libazeban_hip.so once; raise if it is missing."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise AzbError(
+                f"{LIB_PATH} is not built; run `python -m azeban_amd.build` (hipcc, gfx950)")
+        lib = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+def check(rc: int) -> None:
+    if rc != 0:
+        msg = load().azb_last_error()
+        raise AzbError(f"azeban_hip error {rc}: {msg.decode() if msg else ''}")

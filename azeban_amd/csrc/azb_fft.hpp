@@ -1,0 +1,380 @@
+// Device-side FFT building blocks for gfx950: complex type, small-radix DFTs in
+// registers, and a workgroup-wide Stockham FFT whose inter-pass exchange goes
+// through LDS.
+//
+// A "block FFT" transforms TS independent length-L sequences held by NT threads
+// (E = TS*L/NT complex values per thread).  Before the pass of radix R, thread t
+// holds, for every butterfly b in [0, E/R):
+//     g = t + NT*b,  s = g % TS (sequence),  j = g / TS (butterfly),
+//     v[b*R + r] = x_s[j + r*L/R],  r in [0, R).
+// After the last pass the same formula (with the last radix) names the natural-
+// order outputs each thread holds, so callers load the first pass straight from
+// HBM and store the last pass straight to HBM without an LDS round trip.
+//
+// Stockham autosort step (pass with accumulated size Ns, radix R):
+//     k = j mod Ns,  v_r *= W_{Ns R}^{r k},  V = DFT_R(v),
+//     y[(j - k) R + k + r Ns] = V_r.
+// Twiddles come from a per-length table tw[t] = exp(-2 pi i t / L).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace azb {
+
+template <typename T>
+struct cx {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cx<T> mk(T a, T b) {
+  cx<T> r;
+  r.x = a;
+  r.y = b;
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) {
+  return mk(a.x + b.x, a.y + b.y);
+}
+template <typename T>
+__device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) {
+  return mk(a.x - b.x, a.y - b.y);
+}
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
+  return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+template <typename T>
+__device__ __forceinline__ cx<T> cscale(cx<T> a, T s) {
+  return mk(a.x * s, a.y * s);
+}
+// multiply by i*sgn (sgn = +1 or -1)
+template <int SGN, typename T>
+__device__ __forceinline__ cx<T> mul_i(cx<T> a) {
+  return SGN > 0 ? mk(-a.y, a.x) : mk(a.y, -a.x);
+}
+
+// ---------------------------------------------------------------------------
+// constexpr cos/sin of 2*pi*m/n for compile-time twiddles inside composite radices
+// ---------------------------------------------------------------------------
+constexpr double kPi = 3.141592653589793238462643383279502884;
+
+constexpr double ce_sin_poly(double x) {
+  double x2 = x * x, term = x, sum = x;
+  for (int n = 1; n < 14; ++n) {
+    term *= -x2 / ((2 * n) * (2 * n + 1));
+    sum += term;
+  }
+  return sum;
+}
+constexpr double ce_cos_poly(double x) {
+  double x2 = x * x, term = 1.0, sum = 1.0;
+  for (int n = 1; n < 14; ++n) {
+    term *= -x2 / ((2 * n - 1) * (2 * n));
+    sum += term;
+  }
+  return sum;
+}
+// returns cos(2 pi m / n) (q = 0) or sin(2 pi m / n) (q = 1) via octant reduction
+constexpr double ce_trig(long m, long n, int q) {
+  m %= n;
+  if (m < 0) m += n;
+  // angle = 2 pi m / n ; work in units of n/8 octants using exact integer math
+  long m8 = 8 * m;
+  int oct = (int)(m8 / n);  // 0..7
+  double rem = (double)(m8 - (long)oct * n) / (double)n;  // [0,1)
+  double phi = rem * (kPi / 4);  // within octant
+  double c = 0, s = 0;
+  // angle = oct*pi/4 + phi
+  double cphi = ce_cos_poly(phi), sphi = ce_sin_poly(phi);
+  const double h = 0.70710678118654752440084436210484903928;
+  switch (oct) {
+    case 0: c = cphi; s = sphi; break;
+    case 1: c = h * (cphi - sphi); s = h * (cphi + sphi); break;
+    case 2: c = -sphi; s = cphi; break;
+    case 3: c = -h * (cphi + sphi); s = h * (cphi - sphi); break;
+    case 4: c = -cphi; s = -sphi; break;
+    case 5: c = -h * (cphi - sphi); s = -h * (cphi + sphi); break;
+    case 6: c = sphi; s = -cphi; break;
+    default: c = h * (cphi + sphi); s = -h * (cphi - sphi); break;
+  }
+  return q == 0 ? c : s;
+}
+
+// ---------------------------------------------------------------------------
+// In-register DFTs, natural-order in and out.  DIR = -1 forward, +1 inverse.
+// ---------------------------------------------------------------------------
+template <int R, int DIR, typename T>
+struct DFT;
+
+template <int DIR, typename T>
+struct DFT<1, DIR, T> {
+  __device__ __forceinline__ static void run(cx<T>*) {}
+};
+
+template <int DIR, typename T>
+struct DFT<2, DIR, T> {
+  __device__ __forceinline__ static void run(cx<T>* v) {
+    cx<T> a = v[0], b = v[1];
+    v[0] = a + b;
+    v[1] = a - b;
+  }
+};
+
+template <int DIR, typename T>
+struct DFT<3, DIR, T> {
+  __device__ __forceinline__ static void run(cx<T>* v) {
+    const T c = T(0.86602540378443864676372317075293618);  // sin(2 pi / 3)
+    cx<T> t1 = v[1] + v[2];
+    cx<T> t2 = mk(v[0].x - T(0.5) * t1.x, v[0].y - T(0.5) * t1.y);
+    cx<T> d = v[1] - v[2];
+    cx<T> t3 = mul_i<DIR>(cscale(d, c));
+    v[0] = v[0] + t1;
+    v[1] = t2 + t3;
+    v[2] = t2 - t3;
+  }
+};
+
+template <int DIR, typename T>
+struct DFT<4, DIR, T> {
+  __device__ __forceinline__ static void run(cx<T>* v) {
+    cx<T> a = v[0] + v[2], b = v[0] - v[2];
+    cx<T> c = v[1] + v[3], d = mul_i<DIR>(v[1] - v[3]);
+    v[0] = a + c;
+    v[2] = a - c;
+    v[1] = b + d;
+    v[3] = b - d;
+  }
+};
+
+template <int DIR, typename T>
+struct DFT<8, DIR, T> {
+  __device__ __forceinline__ static void run(cx<T>* v) {
+    const T h = T(0.70710678118654752440084436210484903928);
+    cx<T> e[4] = {v[0], v[2], v[4], v[6]};
+    cx<T> o[4] = {v[1], v[3], v[5], v[7]};
+    DFT<4, DIR, T>::run(e);
+    DFT<4, DIR, T>::run(o);
+    // W8^1 = h(1 + DIR i), W8^2 = DIR i, W8^3 = h(-1 + DIR i)
+    cx<T> o1 = cscale(o[1] + mul_i<DIR>(o[1]), h);
+    cx<T> o2 = mul_i<DIR>(o[2]);
+    cx<T> o3 = cscale(mul_i<DIR>(o[3]) - o[3], h);
+    v[0] = e[0] + o[0];
+    v[4] = e[0] - o[0];
+    v[1] = e[1] + o1;
+    v[5] = e[1] - o1;
+    v[2] = e[2] + o2;
+    v[6] = e[2] - o2;
+    v[3] = e[3] + o3;
+    v[7] = e[3] - o3;
+  }
+};
+
+// compile-time table of exp(DIR 2 pi i e / R), e in [0, R)
+template <int R, int DIR>
+struct RootTable {
+  double c[R];
+  double s[R];
+  constexpr RootTable() : c(), s() {
+    for (int e = 0; e < R; ++e) {
+      c[e] = ce_trig(e, R, 0);
+      s[e] = DIR * ce_trig(e, R, 1);
+    }
+  }
+};
+
+// Composite radix R = P*Q: n = Q p + q, k = k1 + P k2
+//   X[k1 + P k2] = sum_q W_Q^{q k2} W_R^{q k1} sum_p x[Q p + q] W_P^{p k1}
+template <int P, int Q, int DIR, typename T>
+struct DFTComposite {
+  __device__ __forceinline__ static void run(cx<T>* v) {
+    constexpr int R = P * Q;
+    constexpr RootTable<R, DIR> tab{};
+    cx<T> y[Q][P];
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+#pragma unroll
+      for (int p = 0; p < P; ++p) y[q][p] = v[Q * p + q];
+      DFT<P, DIR, T>::run(y[q]);
+#pragma unroll
+      for (int k1 = 1; k1 < P; ++k1) {
+        if (q != 0) y[q][k1] = cmul(y[q][k1], mk(T(tab.c[(q * k1) % R]), T(tab.s[(q * k1) % R])));
+      }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) {
+      cx<T> z[Q];
+#pragma unroll
+      for (int q = 0; q < Q; ++q) z[q] = y[q][k1];
+      DFT<Q, DIR, T>::run(z);
+#pragma unroll
+      for (int k2 = 0; k2 < Q; ++k2) v[k1 + P * k2] = z[k2];
+    }
+  }
+};
+
+template <int DIR, typename T>
+struct DFT<6, DIR, T> : DFTComposite<2, 3, DIR, T> {};
+template <int DIR, typename T>
+struct DFT<12, DIR, T> : DFTComposite<4, 3, DIR, T> {};
+template <int DIR, typename T>
+struct DFT<16, DIR, T> : DFTComposite<4, 4, DIR, T> {};
+template <int DIR, typename T>
+struct DFT<24, DIR, T> : DFTComposite<8, 3, DIR, T> {};
+
+// ---------------------------------------------------------------------------
+// Workgroup Stockham FFT
+// ---------------------------------------------------------------------------
+template <int... Rs>
+struct Plan {};
+
+template <int R0, int... Rs>
+struct PlanFirst {
+  static constexpr int value = R0;
+};
+template <int R0>
+constexpr int plan_last() { return R0; }
+template <int R0, int R1, int... Rs>
+constexpr int plan_last() { return plan_last<R1, Rs...>(); }
+template <int R0>
+constexpr long plan_prod() { return R0; }
+template <int R0, int R1, int... Rs>
+constexpr long plan_prod() { return (long)R0 * plan_prod<R1, Rs...>(); }
+
+// LDS words needed by the exchange (SPLIT: one T per element, else two)
+template <typename T, int L, int TS, bool SPLIT>
+struct ExchangeSize {
+  static constexpr int words = TS * L * (SPLIT ? 1 : 2);
+};
+
+template <typename T, int L, int TS, int NT, int DIR, bool SPLIT>
+struct BlockFFT {
+  static constexpr int E = TS * L / NT;
+  static_assert(TS * L % NT == 0, "TS*L must be a multiple of NT");
+
+  // twiddle W_L^{e} with direction applied
+  __device__ __forceinline__ static cx<T> twiddle(const cx<T>* __restrict__ tw, int e) {
+    cx<T> w = tw[e];
+    if (DIR > 0) w.y = -w.y;
+    return w;
+  }
+
+  template <int Ns, int R>
+  __device__ __forceinline__ static void butterflies(cx<T>* v, int t, const cx<T>* __restrict__ tw) {
+    static_assert(E % R == 0, "radix must divide E");
+#pragma unroll
+    for (int b = 0; b < E / R; ++b) {
+      cx<T>* w = v + b * R;
+      if (Ns > 1) {
+        const int g = t + NT * b;
+        const int j = g / TS;
+        const int k = j % Ns;
+        constexpr int step = L / (Ns * R);
+#pragma unroll
+        for (int r = 1; r < R; ++r) w[r] = cmul(w[r], twiddle(tw, r * k * step));
+      }
+      DFT<R, DIR, T>::run(w);
+    }
+  }
+
+  // write pass (Ns,R) outputs into LDS in [pos][s] order, then read the next pass's inputs
+  template <int Ns, int R, int Rn>
+  __device__ __forceinline__ static void exchange(cx<T>* v, int t, T* lds, bool active) {
+    constexpr int comps = SPLIT ? 2 : 1;
+#pragma unroll
+    for (int c = 0; c < comps; ++c) {
+      __syncthreads();
+      if (active) {
+#pragma unroll
+        for (int b = 0; b < E / R; ++b) {
+          const int g = t + NT * b;
+          const int s = g % TS;
+          const int j = g / TS;
+          const int k = j % Ns;
+          const int base = (j - k) * R + k;
+#pragma unroll
+          for (int r = 0; r < R; ++r) {
+            const int idx = (base + r * Ns) * TS + s;
+            if (SPLIT) {
+              lds[idx] = c == 0 ? v[b * R + r].x : v[b * R + r].y;
+            } else {
+              lds[2 * idx] = v[b * R + r].x;
+              lds[2 * idx + 1] = v[b * R + r].y;
+            }
+          }
+        }
+      }
+      __syncthreads();
+      if (active) {
+#pragma unroll
+        for (int b = 0; b < E / Rn; ++b) {
+          const int g = t + NT * b;
+          const int s = g % TS;
+          const int j = g / TS;
+#pragma unroll
+          for (int r = 0; r < Rn; ++r) {
+            const int idx = (j + r * (L / Rn)) * TS + s;
+            if (SPLIT) {
+              if (c == 0)
+                v[b * Rn + r].x = lds[idx];
+              else
+                v[b * Rn + r].y = lds[idx];
+            } else {
+              v[b * Rn + r].x = lds[2 * idx];
+              v[b * Rn + r].y = lds[2 * idx + 1];
+            }
+          }
+        }
+      }
+    }
+  }
+
+  template <int Ns, int R>
+  __device__ __forceinline__ static void passes(cx<T>* v, int t, T* lds, const cx<T>* tw,
+                                                bool active, Plan<R>) {
+    if (active) butterflies<Ns, R>(v, t, tw);
+  }
+  template <int Ns, int R, int Rn, int... Rs>
+  __device__ __forceinline__ static void passes(cx<T>* v, int t, T* lds, const cx<T>* tw,
+                                                bool active, Plan<R, Rn, Rs...>) {
+    if (active) butterflies<Ns, R>(v, t, tw);
+    exchange<Ns, R, Rn>(v, t, lds, active);
+    passes<Ns * R, Rn, Rs...>(v, t, lds, tw, active, Plan<Rn, Rs...>{});
+  }
+
+  // Run the whole transform.  All threads of the workgroup must call this (it holds
+  // barriers); threads with active == false only take part in the barriers.
+  template <int... Rs>
+  __device__ __forceinline__ static void run(cx<T>* v, int t, T* lds, const cx<T>* tw, bool active,
+                                             Plan<Rs...> p) {
+    static_assert(plan_prod<Rs...>() == L, "plan must multiply to L");
+    passes<1, Rs...>(v, t, lds, tw, active, p);
+  }
+
+  // first-pass load: ld(s, pos) -> cx<T>
+  template <int R0, class Ld>
+  __device__ __forceinline__ static void load(cx<T>* v, int t, Ld ld) {
+#pragma unroll
+    for (int b = 0; b < E / R0; ++b) {
+      const int g = t + NT * b;
+      const int s = g % TS;
+      const int j = g / TS;
+#pragma unroll
+      for (int r = 0; r < R0; ++r) v[b * R0 + r] = ld(s, j + r * (L / R0));
+    }
+  }
+  // last-pass store: st(s, pos, value)
+  template <int RL, class St>
+  __device__ __forceinline__ static void store(const cx<T>* v, int t, St st) {
+#pragma unroll
+    for (int b = 0; b < E / RL; ++b) {
+      const int g = t + NT * b;
+      const int s = g % TS;
+      const int j = g / TS;
+#pragma unroll
+      for (int r = 0; r < RL; ++r) st(s, j + r * (L / RL), v[b * RL + r]);
+    }
+  }
+};
+
+}  // namespace azb

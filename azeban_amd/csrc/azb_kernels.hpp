@@ -1,0 +1,749 @@
+// Kernels of the spectral-viscosity Euler right-hand side and the SSP-RK3 stage
+// updates.  One RHS evaluation (IncompressibleEuler::dudt) is, on the M = 3N/2
+// dealiasing grid:
+//
+//   3D: K1 col  inverse axis 0, rows N -> M (zero padding implicit)
+//       K2 col  inverse axis 1, rows N -> M
+//       K3 row  c2r along axis 2, B = n^2 u_i u_j, max|u|, r2c along axis 2,
+//               only the k <= N/2 outputs kept (truncation implicit)
+//       K4 col  forward axis 1, rows M -> N kept
+//       K5 col  forward axis 0, rows M -> N kept  => B_hat on the N-grid
+//       K6 k-space: i k.B_hat, Leray projection, SmoothCutoff viscosity, fused with
+//               the RK stage update
+//   2D: K1 col axis 0, K2 row axis 1, K3 col axis 0, K4 k-space.
+//
+// Reference behaviour restated here: copy_to_padded (copy_to_padded_functions.hpp:
+// 11-63), the FFTW/cuFFT c2r convention (complex passes on the leading axes, then
+// c2r on the last axis dropping Im at k_last = 0), computeB (incompressible_euler.hpp:
+// 168-228, incompressible_euler_functions.hpp:26-45), max_norm (norm.hpp:72-91),
+// computeDudt (incompressible_euler.hpp:231-347, incompressible_euler_functions.hpp:
+// 47-112), SmoothCutoff1D (spectral_viscosity.hpp:56-77) and SSP_RK3 (ssp_rk3.hpp:48-64).
+#pragma once
+#include "azb_fft.hpp"
+
+namespace azb {
+
+constexpr int kBins = 64;
+
+// Device-resident scalars of one solver.  The time loop keeps t, dt and the CFL
+// maximum on the device so a whole step (or many) runs without a host round trip.
+struct DevState {
+  unsigned long long umax_bins[4][kBins];  // per-RHS-slot partial max |u| (as double bits)
+  double u_max;        // max|u| / M^d of the latest RHS evaluation
+  double dt;           // dt of the current step
+  double max_dt;       // clamp of the current step
+  double time;         // simulation time (device-driven loop)
+  double t_end;        // target time of the device-driven loop
+  double max_dt_visc;  // C * 2 / (eps (pi N)^2)
+  int done;            // set when time >= t_end: all step kernels return at once
+  int steps;
+  int pad[2];
+};
+
+enum ColMode { COL_PAD_IN = 0, COL_TRUNC_OUT = 1, COL_FULL = 2 };
+enum KMode { K_DUDT = 0, K_RK1 = 1, K_RK2 = 2, K_RK3 = 3 };
+
+// E (complex values per thread) and radix plan per transform length
+template <int L>
+struct LenPlan;
+template <> struct LenPlan<48> { static constexpr int E = 24; using P = Plan<4, 12>; };
+template <> struct LenPlan<96> { static constexpr int E = 24; using P = Plan<8, 12>; };
+template <> struct LenPlan<192> { static constexpr int E = 24; using P = Plan<8, 24>; };
+template <> struct LenPlan<384> { static constexpr int E = 24; using P = Plan<8, 4, 12>; };
+template <> struct LenPlan<768> { static constexpr int E = 24; using P = Plan<8, 8, 12>; };
+template <> struct LenPlan<1536> { static constexpr int E = 24; using P = Plan<8, 8, 24>; };
+template <> struct LenPlan<16> { static constexpr int E = 16; using P = Plan<16>; };
+template <> struct LenPlan<32> { static constexpr int E = 16; using P = Plan<8, 4>; };
+template <> struct LenPlan<64> { static constexpr int E = 16; using P = Plan<8, 8>; };
+template <> struct LenPlan<128> { static constexpr int E = 16; using P = Plan<8, 16>; };
+template <> struct LenPlan<256> { static constexpr int E = 16; using P = Plan<16, 16>; };
+template <> struct LenPlan<512> { static constexpr int E = 16; using P = Plan<8, 8, 8>; };
+template <> struct LenPlan<1024> { static constexpr int E = 16; using P = Plan<16, 8, 8>; };
+
+template <int... Rs>
+constexpr int first_radix(Plan<Rs...>) { return PlanFirst<Rs...>::value; }
+template <int... Rs>
+constexpr int last_radix(Plan<Rs...>) { return plan_last<Rs...>(); }
+
+// ---------------------------------------------------------------------------
+// Column FFT: batched complex FFT of length L along a strided axis.  The
+// workgroup owns TS consecutive columns (contiguous in memory) of one batch and
+// all L rows.  PAD_IN reads only the N nonzero rows of the zero-padded input;
+// TRUNC_OUT writes only the N rows the N-grid keeps.
+// ---------------------------------------------------------------------------
+struct ColArgs {
+  const void* in;
+  void* out;
+  long ncols;     // columns per batch
+  int tiles;      // ceil(ncols / TS)
+  int nsmall;     // N (unpadded length of the pruned side)
+  long in_rs, out_rs;   // row strides (elements)
+  long in_bs, out_bs;   // batch strides (elements)
+  const void* tw;       // twiddles for L
+  const DevState* st;
+};
+
+template <typename Tin, typename Tout, typename T, int L, int TS, int DIR, int MODE>
+__global__ __launch_bounds__(TS * L / LenPlan<L>::E) void col_fft_kernel(ColArgs a) {
+  using LP = LenPlan<L>;
+  constexpr int E = LP::E;
+  constexpr int NT = TS * L / E;
+  constexpr bool SPLIT = sizeof(T) == 8;
+  using F = BlockFFT<T, L, TS, NT, DIR, SPLIT>;
+  constexpr int R0 = first_radix(typename LP::P{});
+  constexpr int RL = last_radix(typename LP::P{});
+  __shared__ T lds[ExchangeSize<T, L, TS, SPLIT>::words];
+  if (a.st->done) return;
+
+  const int t = threadIdx.x;
+  const int tile = blockIdx.x % a.tiles;
+  const long batch = blockIdx.x / a.tiles;
+  const long c0 = (long)tile * TS;
+  const int N = a.nsmall;
+  const int half = N / 2;
+  const cx<Tin>* __restrict__ in = (const cx<Tin>*)a.in + batch * a.in_bs + c0;
+  cx<Tout>* __restrict__ out = (cx<Tout>*)a.out + batch * a.out_bs + c0;
+  const long ncols_left = a.ncols - c0;
+  const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
+
+  cx<T> v[E];
+  F::template load<R0>(v, t, [&](int s, int pos) -> cx<T> {
+    long row = pos;
+    if (MODE == COL_PAD_IN) {
+      if (pos <= half)
+        row = pos;
+      else if (pos >= L - half + 1)
+        row = pos - (L - N);
+      else
+        row = -1;
+    }
+    cx<T> r = mk(T(0), T(0));
+    if (row >= 0 && s < ncols_left) {
+      cx<Tin> x = in[row * a.in_rs + s];
+      r = mk(T(x.x), T(x.y));
+    }
+    return r;
+  });
+  F::run(v, t, lds, tw, true, typename LP::P{});
+  F::template store<RL>(v, t, [&](int s, int pos, cx<T> val) {
+    long row = pos;
+    if (MODE == COL_TRUNC_OUT) {
+      if (pos <= half)
+        row = pos;
+      else if (pos >= L - half + 1)
+        row = pos - (L - N);
+      else
+        row = -1;
+    }
+    if (row >= 0 && s < ncols_left) {
+      cx<Tout> o;
+      o.x = Tout(val.x);
+      o.y = Tout(val.y);
+      out[row * a.out_rs + s] = o;
+    }
+  });
+}
+
+// ---------------------------------------------------------------------------
+// Row kernel: the last axis of the RHS in one pass.  Each workgroup owns P pairs
+// of lines (a, b).  Inverse: the N/2+1 stored modes of each component are
+// Hermitian-extended onto the M-point line (Im of the k=0 entry dropped, as the
+// reference's c2r does) and two real lines are packed per complex FFT.  The
+// products B_m are formed in registers, packed two lines per forward FFT, and
+// separated into the two half spectra; only k <= N/2 is written.
+// ---------------------------------------------------------------------------
+struct RowArgs {
+  const void* in;   // [c][line][Nh]
+  void* out;        // [m][line][Nh]
+  long nlines;      // lines per component (M for 2D, M*M for 3D)
+  double norm;      // 1 / (N^{d/2} M^{d/2})
+  const void* tw;
+  DevState* st;
+  int slot;         // umax slot
+};
+
+template <typename T, int L, int P, int DIM>
+struct RowCfg {
+  static constexpr int E = LenPlan<L>::E;
+  static constexpr int NINV = DIM == 3 ? 3 : 2;
+  static constexpr int NROUND = DIM == 3 ? 2 : 1;
+  static constexpr int TSI = NINV * P;
+  static constexpr int TSF = 3 * P;
+  static constexpr int NTI = TSI * L / E;
+  static constexpr int NTF = TSF * L / E;
+  static constexpr int NT = NTI > NTF ? NTI : NTF;
+  static constexpr int N = 2 * L / 3;
+  static constexpr int NH = N / 2 + 1;
+  static constexpr bool SPLIT = sizeof(T) == 8;
+  static constexpr int XW = (TSI > TSF ? TSI : TSF) * L;  // exchange words (split layout)
+  static constexpr int XWORDS = SPLIT ? XW : 2 * XW;
+};
+
+template <typename T>
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+template <typename T, int L, int P, int DIM>
+__global__ __launch_bounds__((RowCfg<T, L, P, DIM>::NT)) void euler_row_kernel(RowArgs a) {
+  using C = RowCfg<T, L, P, DIM>;
+  using LP = LenPlan<L>;
+  constexpr int NT = C::NT, E = C::E, NH = C::NH, N = C::N;
+  constexpr int R0 = first_radix(typename LP::P{});
+  constexpr int RL = last_radix(typename LP::P{});
+  using FI = BlockFFT<T, L, C::TSI, C::NTI, +1, C::SPLIT>;
+  using FF = BlockFFT<T, L, C::TSF, C::NTF, -1, C::SPLIT>;
+  __shared__ cx<T> U[C::TSI * L];
+  __shared__ T X[C::XWORDS];
+  __shared__ double red[NT / 64 + 1];
+  if (a.st->done) return;
+
+  const int t = threadIdx.x;
+  const long line0 = (long)blockIdx.x * (2 * P);
+  const cx<T>* __restrict__ in = (const cx<T>*)a.in;
+  cx<T>* __restrict__ out = (cx<T>*)a.out;
+  const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
+  const T nrm = T(a.norm);
+
+  // ---- stage the inputs: stg[(c*2P + l)*NH + k]
+  {
+    cx<T>* stg = U;
+    constexpr int total = DIM * 2 * P * NH;
+    for (int e = t; e < total; e += NT) {
+      const int k = e % NH;
+      const int cl = e / NH;
+      const int l = cl % (2 * P);
+      const int c = cl / (2 * P);
+      stg[e] = in[((long)c * a.nlines + line0 + l) * NH + k];
+    }
+  }
+  __syncthreads();
+
+  // ---- inverse: packed Hermitian-extended lines
+  cx<T> v[E];
+  const bool act_i = t < C::NTI;
+  if (act_i) {
+    FI::template load<R0>(v, t, [&](int s, int pos) -> cx<T> {
+      const int pair = s / C::NINV, type = s % C::NINV;
+      const int la = 2 * pair, lb = 2 * pair + 1;
+      int cx_c, cx_l, cy_c, cy_l;
+      if (DIM == 3) {
+        if (type == 0) { cx_c = 0; cx_l = la; cy_c = 1; cy_l = la; }
+        else if (type == 1) { cx_c = 2; cx_l = la; cy_c = 2; cy_l = lb; }
+        else { cx_c = 0; cx_l = lb; cy_c = 1; cy_l = lb; }
+      } else {
+        if (type == 0) { cx_c = 0; cx_l = la; cy_c = 1; cy_l = la; }
+        else { cx_c = 0; cx_l = lb; cy_c = 1; cy_l = lb; }
+      }
+      const cx<T>* xs = U + (cx_c * 2 * P + cx_l) * NH;
+      const cx<T>* ys = U + (cy_c * 2 * P + cy_l) * NH;
+      if (pos == 0) return mk(xs[0].x, ys[0].x);
+      if (pos <= N / 2) {
+        const cx<T> X0 = xs[pos], Y0 = ys[pos];
+        return mk(X0.x - Y0.y, X0.y + Y0.x);
+      }
+      if (pos >= L - N / 2) {
+        const cx<T> X0 = xs[L - pos], Y0 = ys[L - pos];
+        return mk(X0.x + Y0.y, Y0.x - X0.y);
+      }
+      return mk(T(0), T(0));
+    });
+  }
+  __syncthreads();  // all staging reads done before U is overwritten
+  FI::run(v, t, X, tw, act_i, typename LP::P{});
+  if (act_i) {
+    FI::template store<RL>(v, t, [&](int s, int pos, cx<T> val) { U[s * L + pos] = val; });
+  }
+  __syncthreads();
+
+  double lmax = 0.0;
+#pragma unroll
+  for (int round = 0; round < C::NROUND; ++round) {
+    const bool act_f = t < C::NTF;
+    if (act_f) {
+      FF::template load<R0>(v, t, [&](int s, int pos) -> cx<T> {
+        const int pair = s / 3, type = s % 3;
+        T ua1, ua2, ub1, ub2, ua3 = 0, ub3 = 0;
+        if (DIM == 3) {
+          const cx<T> p0 = U[(pair * 3 + 0) * L + pos];
+          const cx<T> p1 = U[(pair * 3 + 1) * L + pos];
+          const cx<T> p2 = U[(pair * 3 + 2) * L + pos];
+          ua1 = p0.x; ua2 = p0.y; ua3 = p1.x; ub3 = p1.y; ub1 = p2.x; ub2 = p2.y;
+        } else {
+          const cx<T> p0 = U[(pair * 2 + 0) * L + pos];
+          const cx<T> p1 = U[(pair * 2 + 1) * L + pos];
+          ua1 = p0.x; ua2 = p0.y; ub1 = p1.x; ub2 = p1.y;
+        }
+        if (round == 0 && type == 1) {
+          lmax = fmax(lmax, (double)fmax(fmax(fabs(ua1), fabs(ua2)), fmax(fabs(ub1), fabs(ub2))));
+        }
+        if (DIM == 3 && round == 1 && type == 2) {
+          lmax = fmax(lmax, (double)fmax(fabs(ua3), fabs(ub3)));
+        }
+        const T a1 = nrm * ua1, a2 = nrm * ua2, a3 = nrm * ua3;
+        const T b1 = nrm * ub1, b2 = nrm * ub2, b3 = nrm * ub3;
+        const int m = round * 3 + type;
+        if (DIM == 3) {
+          // (11, 21, 22, 31, 32, 33) = (u1u1, u2u1, u2u2, u3u1, u3u2, u3u3)
+          switch (m) {
+            case 0: return mk(a1 * a1, b1 * b1);
+            case 1: return mk(a2 * a1, b2 * b1);
+            case 2: return mk(a2 * a2, b2 * b2);
+            case 3: return mk(a3 * a1, b3 * b1);
+            case 4: return mk(a3 * a2, b3 * b2);
+            default: return mk(a3 * a3, b3 * b3);
+          }
+        } else {
+          switch (m) {
+            case 0: return mk(a1 * a1, b1 * b1);
+            case 1: return mk(a1 * a2, b1 * b2);
+            default: return mk(a2 * a2, b2 * b2);
+          }
+        }
+      });
+    }
+    FF::run(v, t, X, tw, act_f, typename LP::P{});
+    // separate the two half spectra: A = (Z[k] + conj Z[-k]) / 2, B = (Z[k] - conj Z[-k]) / 2i
+    constexpr int items = C::TSF * NH;
+    constexpr int per = (items + NT - 1) / NT;
+    T ra[per], rb[per], ia[per], ib[per];
+#pragma unroll
+    for (int ph = 0; ph < 2; ++ph) {
+      __syncthreads();
+      if (act_f) {
+        FF::template store<RL>(v, t, [&](int s, int pos, cx<T> val) {
+          X[s * L + pos] = ph == 0 ? val.x : val.y;
+        });
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < per; ++q) {
+        const int it = t + NT * q;
+        if (it < items) {
+          const int k = it % NH, s = it / NH;
+          const T zk = X[s * L + k];
+          const T zm = X[s * L + (k == 0 ? 0 : L - k)];
+          if (ph == 0) {
+            ra[q] = T(0.5) * (zk + zm);
+            ib[q] = T(-0.5) * (zk - zm);
+          } else {
+            ia[q] = T(0.5) * (zk - zm);
+            rb[q] = T(0.5) * (zk + zm);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < per; ++q) {
+      const int it = t + NT * q;
+      if (it < items) {
+        const int k = it % NH, s = it / NH;
+        const int pair = s / 3, type = s % 3;
+        const int m = round * 3 + type;
+        const long la = line0 + 2 * pair, lb = la + 1;
+        out[((long)m * a.nlines + la) * NH + k] = mk(ra[q], ia[q]);
+        out[((long)m * a.nlines + lb) * NH + k] = mk(rb[q], ib[q]);
+      }
+    }
+  }
+
+  // ---- block max -> one atomic per workgroup into a spread bin
+  lmax = wave_max<T>(lmax);
+  if ((t & 63) == 0) red[t >> 6] = lmax;
+  __syncthreads();
+  if (t == 0) {
+    double m = 0.0;
+    for (int w = 0; w < (NT + 63) / 64; ++w) m = fmax(m, red[w]);
+    atomicMax(&a.st->umax_bins[a.slot][blockIdx.x & (kBins - 1)],
+              (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// K-space kernel with the RK stage update fused.
+// ---------------------------------------------------------------------------
+struct KArgs {
+  const void* bh;        // [m][N-grid] at working precision
+  const double2* u_in;   // stage input u_hat
+  const double2* u0;     // RK base state (stages 2, 3)
+  double2* out;
+  long nmodes;           // modes per component
+  int n, nh;
+  double eps, s2, k0;    // SmoothCutoff1D with s2 = 2s
+  int mode;
+  const DevState* st;
+};
+
+__device__ __forceinline__ double sv_eval(double k, double eps, double s2, double k0) {
+  const double k1 = fabs(k) / k0;
+  const double k2 = k1 * k1;
+  const double k4 = k2 * k2;
+  const double k8 = k4 * k4;
+  const double k16 = k8 * k8;
+  const double k18 = k16 * k2;
+  const double qk = 1 - exp(-k18);
+  const double kp = s2 == 2.0 ? k * k : pow(k, s2);
+  return -eps * kp * qk;
+}
+
+__device__ __forceinline__ double2 cmuli(double k, double2 b) {
+  // complex_t(0, k) * b
+  return make_double2(0.0 * b.x - k * b.y, 0.0 * b.y + k * b.x);
+}
+
+template <typename T, int DIM>
+__global__ __launch_bounds__(256) void kspace_kernel(KArgs a) {
+  if (a.st->done) return;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.nmodes) return;
+  const double twopi = 2.0 * 3.14159265358979323846;
+  const int N = a.n, NH = a.nh;
+  const cx<T>* bh = (const cx<T>*)a.bh;
+  auto B = [&](int m) -> double2 {
+    const cx<T> z = bh[(long)m * a.nmodes + idx];
+    return make_double2((double)z.x, (double)z.y);
+  };
+  const int kk = (int)(idx % NH);
+  long rest = idx / NH;
+  double absk2;
+  double2 L[3];
+  if (DIM == 2) {
+    const int i = (int)rest;
+    const int i_ = i >= N / 2 + 1 ? i - N : i;
+    const double k1 = twopi * i_;
+    const double k2 = twopi * kk;
+    absk2 = k1 * k1 + k2 * k2;
+    const double2 B11 = B(0), B12 = B(1), B22 = B(2);
+    const double2 t1 = cmuli(k1, B11), t2 = cmuli(k2, B12);
+    const double2 b1 = make_double2(t1.x + t2.x, t1.y + t2.y);
+    const double2 t3 = cmuli(k1, B12), t4 = cmuli(k2, B22);
+    const double2 b2 = make_double2(t3.x + t4.x, t3.y + t4.y);
+    const double inv = 1.0 / absk2;
+    const double k11 = k1 * k1 * inv, k12 = k1 * k2 * inv, k22 = k2 * k2 * inv;
+    const double c11 = 1.0 - k11, c12 = 0.0 - k12, c22 = 1.0 - k22;
+    L[0] = make_double2(c11 * b1.x + c12 * b2.x, c11 * b1.y + c12 * b2.y);
+    L[1] = make_double2(c12 * b1.x + c22 * b2.x, c12 * b1.y + c22 * b2.y);
+  } else {
+    const int j = (int)(rest % N);
+    const int i = (int)(rest / N);
+    const int i_ = i >= N / 2 + 1 ? i - N : i;
+    const int j_ = j >= N / 2 + 1 ? j - N : j;
+    const double k1 = twopi * i_, k2 = twopi * j_, k3 = twopi * kk;
+    absk2 = k1 * k1 + k2 * k2 + k3 * k3;
+    const double2 B11 = B(0), B21 = B(1), B22 = B(2), B31 = B(3), B32 = B(4), B33 = B(5);
+    double2 b[3];
+    {
+      const double2 x = cmuli(k1, B11), y = cmuli(k2, B21), z = cmuli(k3, B31);
+      b[0] = make_double2(x.x + y.x + z.x, x.y + y.y + z.y);
+    }
+    {
+      const double2 x = cmuli(k1, B21), y = cmuli(k2, B22), z = cmuli(k3, B32);
+      b[1] = make_double2(x.x + y.x + z.x, x.y + y.y + z.y);
+    }
+    {
+      const double2 x = cmuli(k1, B31), y = cmuli(k2, B32), z = cmuli(k3, B33);
+      b[2] = make_double2(x.x + y.x + z.x, x.y + y.y + z.y);
+    }
+    const double inv = 1.0 / absk2;
+    const double k11 = k1 * k1 * inv, k12 = k1 * k2 * inv, k13 = k1 * k3 * inv;
+    const double k22 = k2 * k2 * inv, k23 = k2 * k3 * inv, k33 = k3 * k3 * inv;
+    const double P[3][3] = {{1.0 - k11, 0.0 - k12, 0.0 - k13},
+                            {0.0 - k12, 1.0 - k22, 0.0 - k23},
+                            {0.0 - k13, 0.0 - k23, 1.0 - k33}};
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+      L[r] = make_double2(P[r][0] * b[0].x + P[r][1] * b[1].x + P[r][2] * b[2].x,
+                          P[r][0] * b[0].y + P[r][1] * b[1].y + P[r][2] * b[2].y);
+    }
+  }
+  const double nu = sv_eval(sqrt(absk2), a.eps, a.s2, a.k0);
+  const double dt = a.st->dt;
+#pragma unroll
+  for (int c = 0; c < DIM; ++c) {
+    const long o = (long)c * a.nmodes + idx;
+    const double2 u = a.u_in[o];
+    double2 d = absk2 == 0 ? make_double2(0.0, 0.0)
+                           : make_double2(-L[c].x + nu * u.x, -L[c].y + nu * u.y);
+    double2 r;
+    if (a.mode == K_DUDT) {
+      r = d;
+    } else if (a.mode == K_RK1) {
+      r = make_double2(1.0 * u.x + dt * d.x, 1.0 * u.y + dt * d.y);
+    } else {
+      const double2 tmp = make_double2(1.0 * u.x + dt * d.x, 1.0 * u.y + dt * d.y);
+      const double2 w = a.u0[o];
+      if (a.mode == K_RK2) {
+        r = make_double2(0.75 * w.x + 0.25 * tmp.x, 0.75 * w.y + 0.25 * tmp.y);
+      } else {
+        r = make_double2((2.0 / 3) * tmp.x + (1.0 / 3) * w.x, (2.0 / 3) * tmp.y + (1.0 / 3) * w.y);
+      }
+    }
+    a.out[o] = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Small control kernels (single workgroup)
+// ---------------------------------------------------------------------------
+// clear the per-slot max bins; fixed-step mode sets max_dt from the host value
+__global__ void step_prologue_fixed(DevState* st, double max_dt) {
+  const int t = threadIdx.x;
+  for (int e = t; e < 4 * kBins; e += blockDim.x) (&st->umax_bins[0][0])[e] = 0ull;
+  if (t == 0) {
+    st->done = 0;
+    st->max_dt = max_dt;
+  }
+}
+
+// device-driven loop: a step past t_end does nothing (done = 1)
+__global__ void step_prologue_loop(DevState* st) {
+  const int t = threadIdx.x;
+  const bool go = st->time < st->t_end;
+  for (int e = t; e < 4 * kBins; e += blockDim.x) (&st->umax_bins[0][0])[e] = 0ull;
+  if (t == 0) {
+    st->done = go ? 0 : 1;
+    if (go) st->max_dt = fmin(st->t_end - st->time, st->max_dt_visc);
+  }
+}
+
+__global__ void step_epilogue_loop(DevState* st) {
+  if (threadIdx.x == 0 && !st->done) {
+    st->time += st->dt;
+    st->steps += 1;
+  }
+}
+
+// reduce one slot's bins: u_max = max / M^d; stage 1 also sets dt = min(C/(N u_max), max_dt)
+__global__ void finalize_umax(DevState* st, int slot, int set_dt, double Md, double C, double N) {
+  if (st->done) return;
+  const int t = threadIdx.x;  // 64 threads
+  double v = __longlong_as_double((long long)st->umax_bins[slot][t]);
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  if (t == 0) {
+    const double um = v / Md;
+    st->u_max = um;
+    if (set_dt) st->dt = fmin(C / (N * um), st->max_dt);
+  }
+}
+
+}  // namespace azb
+
+namespace azb {
+
+// ---------------------------------------------------------------------------
+// Plain last-axis real transforms (standalone FFT entry points): pairs of real
+// lines share one complex FFT.  c2r drops Im at k = 0 and k = L/2 (FFTW/cuFFT
+// convention); r2c writes the L/2+1 non-negative frequencies.
+// ---------------------------------------------------------------------------
+struct PlainRowArgs {
+  const void* in;
+  void* out;
+  long nlines;
+  const void* tw;
+};
+
+template <typename T, int L, int P, int DIR>
+__global__ __launch_bounds__(2 * P * L / LenPlan<L>::E / 2) void plain_row_kernel(PlainRowArgs a) {
+  // TS = P (one complex FFT per pair of lines)
+  using LP = LenPlan<L>;
+  constexpr int TS = P;
+  constexpr int NT = TS * L / LP::E;
+  constexpr int NH = L / 2 + 1;
+  constexpr bool SPLIT = sizeof(T) == 8;
+  constexpr int R0 = first_radix(typename LP::P{});
+  constexpr int RL = last_radix(typename LP::P{});
+  using F = BlockFFT<T, L, TS, NT, DIR, SPLIT>;
+  __shared__ T X[SPLIT ? TS * L : 2 * TS * L];
+  __shared__ cx<T> stg[DIR > 0 ? 2 * P * NH : 1];
+  const int t = threadIdx.x;
+  const long line0 = (long)blockIdx.x * 2 * P;
+  const cx<T>* tw = (const cx<T>*)a.tw;
+  cx<T> v[LP::E];
+  if (DIR > 0) {
+    const cx<T>* in = (const cx<T>*)a.in;
+    for (int e = t; e < 2 * P * NH; e += NT) {
+      const long line = line0 + e / NH;
+      stg[e] = line < a.nlines ? in[line * NH + e % NH] : mk(T(0), T(0));
+    }
+    __syncthreads();
+    F::template load<R0>(v, t, [&](int s, int pos) -> cx<T> {
+      const cx<T>* xs = stg + (2 * s) * NH;
+      const cx<T>* ys = stg + (2 * s + 1) * NH;
+      if (pos == 0 || pos == L / 2) return mk(xs[pos].x, ys[pos].x);
+      if (pos < L / 2) {
+        const cx<T> X0 = xs[pos], Y0 = ys[pos];
+        return mk(X0.x - Y0.y, X0.y + Y0.x);
+      }
+      const cx<T> X0 = xs[L - pos], Y0 = ys[L - pos];
+      return mk(X0.x + Y0.y, Y0.x - X0.y);
+    });
+    F::run(v, t, X, tw, true, typename LP::P{});
+    T* out = (T*)a.out;
+    F::template store<RL>(v, t, [&](int s, int pos, cx<T> val) {
+      const long la = line0 + 2 * s, lb = la + 1;
+      if (la < a.nlines) out[la * L + pos] = val.x;
+      if (lb < a.nlines) out[lb * L + pos] = val.y;
+    });
+  } else {
+    const T* in = (const T*)a.in;
+    F::template load<R0>(v, t, [&](int s, int pos) -> cx<T> {
+      const long la = line0 + 2 * s, lb = la + 1;
+      const T xa = la < a.nlines ? in[la * L + pos] : T(0);
+      const T xb = lb < a.nlines ? in[lb * L + pos] : T(0);
+      return mk(xa, xb);
+    });
+    F::run(v, t, X, tw, true, typename LP::P{});
+    cx<T>* out = (cx<T>*)a.out;
+    constexpr int items = TS * NH;
+    for (int ph = 0; ph < 2; ++ph) {
+      __syncthreads();
+      F::template store<RL>(v, t, [&](int s, int pos, cx<T> val) {
+        X[s * L + pos] = ph == 0 ? val.x : val.y;
+      });
+      __syncthreads();
+      for (int it = t; it < items; it += NT) {
+        const int k = it % NH, s = it / NH;
+        const T zk = X[s * L + k];
+        const T zm = X[s * L + (k == 0 ? 0 : L - k)];
+        const long la = line0 + 2 * s, lb = la + 1;
+        // stage the first half-result in the output (re-read in phase 1)
+        if (ph == 0) {
+          if (la < a.nlines) out[la * NH + k].x = T(0.5) * (zk + zm);
+          if (lb < a.nlines) out[lb * NH + k].y = T(-0.5) * (zk - zm);
+        } else {
+          if (la < a.nlines) out[la * NH + k].y = T(0.5) * (zk - zm);
+          if (lb < a.nlines) out[lb * NH + k].x = T(0.5) * (zk + zm);
+        }
+      }
+    }
+  }
+}
+
+// zero padding of one component (copy_to_padded_functions.hpp:11-63): full axes
+// keep i <= N/2, move i >= N/2+1 to i + M - N, zero the gap; compact last axis
+// keeps k < N/2+1.
+__global__ void pad_kernel(const double2* in, double2* out, int dim, int N, int M) {
+  const int nh = N / 2 + 1, mh = M / 2 + 1;
+  const long total = dim == 2 ? (long)M * mh : (long)M * M * mh;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  const int k = (int)(idx % mh);
+  long rest = idx / mh;
+  auto src_full = [&](int i) -> int {
+    if (i < N / 2 + 1) return i;
+    if (i < N / 2 + 1 + M - N) return -1;
+    return i + N - M;
+  };
+  double2 r = make_double2(0.0, 0.0);
+  if (dim == 2) {
+    const int i = src_full((int)rest);
+    if (i >= 0 && k < nh) r = in[(long)i * nh + k];
+  } else {
+    const int j = src_full((int)(rest % M));
+    const int i = src_full((int)(rest / M));
+    if (i >= 0 && j >= 0 && k < nh) r = in[((long)i * N + j) * nh + k];
+  }
+  out[idx] = r;
+}
+
+// y <- a x + b y over a flat complex array (axpby.hpp:41-46)
+__global__ void axpby_kernel(double a, const double2* x, double b, double2* y, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double2 xv = x[i], yv = y[i];
+  y[i] = make_double2(a * xv.x + b * yv.x, a * xv.y + b * yv.y);
+}
+
+// max |x| over a flat real array (norm.hpp:72-91) into a device scalar
+__global__ void max_abs_kernel(const double* x, long n, unsigned long long* out) {
+  double m = 0.0;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+    m = fmax(m, fabs(x[i]));
+  for (int o = 32; o > 0; o >>= 1) m = fmax(m, __shfl_xor(m, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, (unsigned long long)__double_as_longlong(m));
+}
+
+}  // namespace azb
+
+namespace azb {
+
+// Leray projection in place, u_hat <- (I - k k^T / |k|^2) u_hat, k = 0 mode set to 0
+// (src/azeban/operations/leray.cpp:28-116).
+__global__ void leray_kernel(double2* u, int dim, int N, long nmodes) {
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= nmodes) return;
+  const int nh = N / 2 + 1;
+  const double twopi = 2.0 * 3.14159265358979323846;
+  const int kk = (int)(idx % nh);
+  const long rest = idx / nh;
+  double k[3];
+  if (dim == 2) {
+    const int i = (int)rest;
+    k[0] = twopi * (i >= nh ? i - N : i);
+    k[1] = twopi * kk;
+    k[2] = 0;
+  } else {
+    const int j = (int)(rest % N), i = (int)(rest / N);
+    k[0] = twopi * (i >= nh ? i - N : i);
+    k[1] = twopi * (j >= nh ? j - N : j);
+    k[2] = twopi * kk;
+  }
+  double absk2 = 0;
+  for (int d = 0; d < dim; ++d) absk2 += k[d] * k[d];
+  double2 v[3];
+  for (int d = 0; d < dim; ++d) v[d] = u[(long)d * nmodes + idx];
+  for (int r = 0; r < dim; ++r) {
+    double2 o = make_double2(0.0, 0.0);
+    if (absk2 != 0) {
+      for (int c = 0; c < dim; ++c) {
+        const double p = (r == c ? 1.0 : 0.0) - (k[r] * k[c]) / absk2;
+        o.x += p * v[c].x;
+        o.y += p * v[c].y;
+      }
+    }
+    u[(long)r * nmodes + idx] = o;
+  }
+}
+
+// Energy spectrum E(K), K = max|k_i| (L-infinity shells), 1/2 |u_hat / N^d|^2 with
+// conjugate doubling for k_last != 0 (src/azeban/operations/energy_spectrum.cpp:10-50,
+// include/azeban/operations/spectrum.hpp:33-118).  Shell sums accumulate in LDS per
+// workgroup, then one double atomicAdd per shell per workgroup.
+__global__ void energy_spectrum_kernel(const double2* u, int dim, int N, long nmodes, double* spec) {
+  extern __shared__ double sh[];
+  const int nh = N / 2 + 1;
+  for (int e = threadIdx.x; e < nh; e += blockDim.x) sh[e] = 0.0;
+  __syncthreads();
+  const double norm = 1.0 / pow((double)N, dim);
+  for (long idx = (long)blockIdx.x * blockDim.x + threadIdx.x; idx < nmodes;
+       idx += (long)gridDim.x * blockDim.x) {
+    const int kk = (int)(idx % nh);
+    const long rest = idx / nh;
+    int K;
+    if (dim == 2) {
+      const int i = (int)rest;
+      const int k1 = abs(i >= nh ? i - N : i);
+      K = max(k1, kk);
+    } else {
+      const int j = (int)(rest % N), i = (int)(rest / N);
+      const int k1 = abs(i >= nh ? i - N : i), k2 = abs(j >= nh ? j - N : j);
+      K = max(max(k1, k2), kk);
+    }
+    double e = 0;
+    for (int d = 0; d < dim; ++d) {
+      const double2 v = u[(long)d * nmodes + idx];
+      const double x = norm * v.x, y = norm * v.y;
+      e += x * x + y * y;
+    }
+    e *= 0.5;
+    if (kk != 0) e *= 2.0;
+    atomicAdd(&sh[K], e);
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < nh; e += blockDim.x)
+    if (sh[e] != 0.0) atomicAdd(&spec[e], sh[e]);
+}
+
+}  // namespace azb

@@ -1,0 +1,614 @@
+// Host side of the C ABI: solver lifetime, workspace layout, kernel launch
+// sequences of one RHS evaluation / one SSP-RK3 step / the device-driven time
+// loop, and the standalone FFT, pad, axpby and max-norm seams.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/azeban_hip.h"
+#include "azb_kernels.hpp"
+
+using namespace azb;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (expr);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return fail(AZB_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+bool good_len(int L) {
+  switch (L) {
+    case 48: case 96: case 192: case 384: case 768: case 1536:
+    case 16: case 32: case 64: case 128: case 256: case 512: case 1024:
+      return true;
+    default:
+      return false;
+  }
+}
+
+// columns per workgroup of the column kernels: 6144 complex values per block
+int col_ts(int L) {
+  int ts = 6144 / L;
+  if (L % 3 != 0) ts = 4096 / L;
+  return ts < 1 ? 1 : ts;
+}
+
+// line pairs per workgroup of the fused row kernel
+int row_p(int L) {
+  int p = 1536 / L;
+  return p < 1 ? 1 : p;
+}
+
+// ---------------------------------------------------------------------------
+// twiddle tables exp(-2 pi i t / L), computed in long double on the host
+// ---------------------------------------------------------------------------
+template <typename T>
+int make_twiddles(int L, void** out) {
+  std::vector<cx<T>> h(L);
+  const long double twopi = 6.283185307179586476925286766559005768L;
+  for (int t = 0; t < L; ++t) {
+    const long double a = twopi * (long double)t / (long double)L;
+    h[t].x = (T)cosl(a);
+    h[t].y = (T)(-sinl(a));
+  }
+  HIPCHK(hipMalloc(out, sizeof(cx<T>) * L));
+  HIPCHK(hipMemcpy(*out, h.data(), sizeof(cx<T>) * L, hipMemcpyHostToDevice));
+  return AZB_OK;
+}
+
+// ---------------------------------------------------------------------------
+// launch dispatch over the compile-time transform length
+// ---------------------------------------------------------------------------
+template <typename Tin, typename Tout, typename T, int L, int DIR, int MODE>
+void launch_col_L(ColArgs a, long batches, hipStream_t st) {
+  constexpr int E = LenPlan<L>::E;
+  constexpr int TS = (L % 3 == 0) ? (6144 / L < 1 ? 1 : 6144 / L) : (4096 / L < 1 ? 1 : 4096 / L);
+  a.tiles = (int)((a.ncols + TS - 1) / TS);
+  const long blocks = batches * a.tiles;
+  hipLaunchKernelGGL((col_fft_kernel<Tin, Tout, T, L, TS, DIR, MODE>), dim3((unsigned)blocks),
+                     dim3(TS * L / E), 0, st, a);
+}
+
+template <typename Tin, typename Tout, typename T, int DIR, int MODE>
+int launch_col(int L, ColArgs a, long batches, hipStream_t st) {
+  switch (L) {
+#define AZB_CASE(LL) \
+  case LL: launch_col_L<Tin, Tout, T, LL, DIR, MODE>(a, batches, st); break;
+    AZB_CASE(96) AZB_CASE(192) AZB_CASE(384) AZB_CASE(768) AZB_CASE(1536)
+#undef AZB_CASE
+    default:
+      return fail(AZB_ERR_UNSUPPORTED, "column FFT length " + std::to_string(L) + " not built");
+  }
+  return AZB_OK;
+}
+
+template <typename T, int DIR>
+int launch_col_full(int L, ColArgs a, long batches, hipStream_t st) {
+  switch (L) {
+#define AZB_CASE(LL) \
+  case LL: launch_col_L<T, T, T, LL, DIR, COL_FULL>(a, batches, st); break;
+    AZB_CASE(16) AZB_CASE(32) AZB_CASE(64) AZB_CASE(128) AZB_CASE(256) AZB_CASE(512)
+    AZB_CASE(1024) AZB_CASE(48) AZB_CASE(96) AZB_CASE(192) AZB_CASE(384) AZB_CASE(768)
+    AZB_CASE(1536)
+#undef AZB_CASE
+    default:
+      return fail(AZB_ERR_UNSUPPORTED, "FFT length " + std::to_string(L) + " not built");
+  }
+  return AZB_OK;
+}
+
+template <typename T, int L, int DIM>
+void launch_row_L(RowArgs a, hipStream_t st) {
+  constexpr int P = (1536 / L) < 1 ? 1 : 1536 / L;
+  using C = RowCfg<T, L, P, DIM>;
+  // every workgroup owns exactly P whole line pairs (checked by the caller)
+  const long blocks = a.nlines / (2 * P);
+  hipLaunchKernelGGL((euler_row_kernel<T, L, P, DIM>), dim3((unsigned)blocks), dim3(C::NT), 0, st, a);
+}
+
+template <typename T, int DIM>
+int launch_row(int L, RowArgs a, hipStream_t st) {
+  const int P = row_p(L);
+  if (a.nlines % (2 * P) != 0) return fail(AZB_ERR_UNSUPPORTED, "line count not a multiple of the line-pair tile");
+  switch (L) {
+#define AZB_CASE(LL) \
+  case LL: launch_row_L<T, LL, DIM>(a, st); break;
+    AZB_CASE(96) AZB_CASE(192) AZB_CASE(384) AZB_CASE(768) AZB_CASE(1536)
+#undef AZB_CASE
+    default:
+      return fail(AZB_ERR_UNSUPPORTED, "row length " + std::to_string(L) + " not built");
+  }
+  return AZB_OK;
+}
+
+template <typename T, int L, int DIR>
+void launch_plain_row_L(PlainRowArgs a, hipStream_t st) {
+  constexpr int P = (L % 3 == 0) ? ((1536 / L) < 1 ? 1 : 1536 / L) : ((1024 / L) < 1 ? 1 : 1024 / L);
+  constexpr int NT = P * L / LenPlan<L>::E;
+  const long blocks = (a.nlines + 2 * P - 1) / (2 * P);
+  hipLaunchKernelGGL((plain_row_kernel<T, L, P, DIR>), dim3((unsigned)blocks), dim3(NT), 0, st, a);
+}
+
+template <typename T, int DIR>
+int launch_plain_row(int L, PlainRowArgs a, hipStream_t st) {
+  switch (L) {
+#define AZB_CASE(LL) \
+  case LL: launch_plain_row_L<T, LL, DIR>(a, st); break;
+    AZB_CASE(16) AZB_CASE(32) AZB_CASE(64) AZB_CASE(128) AZB_CASE(256) AZB_CASE(512)
+    AZB_CASE(1024) AZB_CASE(48) AZB_CASE(96) AZB_CASE(192) AZB_CASE(384) AZB_CASE(768)
+    AZB_CASE(1536)
+#undef AZB_CASE
+    default:
+      return fail(AZB_ERR_UNSUPPORTED, "FFT length " + std::to_string(L) + " not built");
+  }
+  return AZB_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// solver
+// ---------------------------------------------------------------------------
+struct azb_solver {
+  int dim = 0, N = 0, M = 0, nh = 0, nvars = 0, prec = 0;
+  azb_sv_params visc{};
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  size_t tsize = 8;  // bytes of one real at working precision
+  void* X = nullptr;  // T1 / S2
+  void* Y = nullptr;  // T2 / B_hat
+  void* Z = nullptr;  // S1
+  double2* u1 = nullptr;
+  double2* u2 = nullptr;
+  DevState* st = nullptr;
+  void* tw = nullptr;
+  size_t bytes = 0;
+  long nmodes = 0;  // modes per component on the N-grid
+};
+
+namespace {
+
+template <typename T>
+int enqueue_rhs_T(azb_solver* s, const double2* u_in, const double2* u0, double2* out, int kmode,
+                  int slot, double C) {
+  const int N = s->N, M = s->M, nh = s->nh;
+  hipStream_t st = s->stream;
+  int rc;
+  ColArgs c{};
+  c.nsmall = N;
+  c.tw = s->tw;
+  c.st = s->st;
+  RowArgs r{};
+  r.tw = s->tw;
+  r.st = s->st;
+  r.slot = slot;
+  r.norm = 1.0 / (std::pow(std::sqrt((double)N), s->dim) * std::pow(std::sqrt((double)M), s->dim));
+  if (s->dim == 3) {
+    // K1: inverse along axis 0, N -> M rows; [c][i][j][k] -> X[c][i'][j][k]
+    c.in = u_in; c.out = s->X; c.ncols = (long)N * nh;
+    c.in_rs = (long)N * nh; c.out_rs = (long)N * nh;
+    c.in_bs = (long)N * N * nh; c.out_bs = (long)M * N * nh;
+    if ((rc = launch_col<double, T, T, +1, COL_PAD_IN>(M, c, 3, st))) return rc;
+    // K2: inverse along axis 1; X[c][i'][j][k] -> Y[c][i'][j'][k]
+    c.in = s->X; c.out = s->Y; c.ncols = nh;
+    c.in_rs = nh; c.out_rs = nh;
+    c.in_bs = (long)N * nh; c.out_bs = (long)M * nh;
+    if ((rc = launch_col<T, T, T, +1, COL_PAD_IN>(M, c, 3L * M, st))) return rc;
+    // K3: last axis c2r, products, max, r2c; Y[c][line][k] -> Z[m][line][k]
+    r.in = s->Y; r.out = s->Z; r.nlines = (long)M * M;
+    if ((rc = launch_row<T, 3>(M, r, st))) return rc;
+    // K4: forward along axis 1, M -> N rows; Z[m][i'][j'][k] -> X[m][i'][j][k]
+    c.in = s->Z; c.out = s->X; c.ncols = nh;
+    c.in_rs = nh; c.out_rs = nh;
+    c.in_bs = (long)M * nh; c.out_bs = (long)N * nh;
+    if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 6L * M, st))) return rc;
+    // K5: forward along axis 0; X[m][i'][j][k] -> Y[m][i][j][k] (B_hat on the N-grid)
+    c.in = s->X; c.out = s->Y; c.ncols = (long)N * nh;
+    c.in_rs = (long)N * nh; c.out_rs = (long)N * nh;
+    c.in_bs = (long)M * N * nh; c.out_bs = (long)N * N * nh;
+    if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 6, st))) return rc;
+  } else {
+    // K1: inverse along axis 0; [c][i][k] -> X[c][i'][k]
+    c.in = u_in; c.out = s->X; c.ncols = nh;
+    c.in_rs = nh; c.out_rs = nh;
+    c.in_bs = (long)N * nh; c.out_bs = (long)M * nh;
+    if ((rc = launch_col<double, T, T, +1, COL_PAD_IN>(M, c, 2, st))) return rc;
+    // K2: last axis; X[c][i'][k] -> Z[m][i'][k]
+    r.in = s->X; r.out = s->Z; r.nlines = M;
+    if ((rc = launch_row<T, 2>(M, r, st))) return rc;
+    // K3: forward along axis 0; Z[m][i'][k] -> Y[m][i][k]
+    c.in = s->Z; c.out = s->Y; c.ncols = nh;
+    c.in_rs = nh; c.out_rs = nh;
+    c.in_bs = (long)M * nh; c.out_bs = (long)N * nh;
+    if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 3, st))) return rc;
+  }
+  const double Md = std::pow((double)M, s->dim);
+  hipLaunchKernelGGL(finalize_umax, dim3(1), dim3(64), 0, st, s->st, slot, kmode == K_RK1 ? 1 : 0,
+                     Md, C, (double)N);
+  KArgs k{};
+  k.bh = s->Y;
+  k.u_in = u_in;
+  k.u0 = u0;
+  k.out = out;
+  k.nmodes = s->nmodes;
+  k.n = N;
+  k.nh = nh;
+  k.eps = s->visc.eps;
+  k.s2 = 2 * s->visc.s;
+  k.k0 = s->visc.k0;
+  k.mode = kmode;
+  k.st = s->st;
+  const unsigned blocks = (unsigned)((s->nmodes + 255) / 256);
+  if (s->dim == 3)
+    hipLaunchKernelGGL((kspace_kernel<T, 3>), dim3(blocks), dim3(256), 0, st, k);
+  else
+    hipLaunchKernelGGL((kspace_kernel<T, 2>), dim3(blocks), dim3(256), 0, st, k);
+  HIPCHK(hipGetLastError());
+  return AZB_OK;
+}
+
+int enqueue_rhs(azb_solver* s, const double2* u_in, const double2* u0, double2* out, int kmode,
+                int slot, double C) {
+  if (s->prec == AZB_FP64) return enqueue_rhs_T<double>(s, u_in, u0, out, kmode, slot, C);
+  return enqueue_rhs_T<float>(s, u_in, u0, out, kmode, slot, C);
+}
+
+// the three stages of SSP_RK3::integrate (ssp_rk3.hpp:54-62), stage updates fused
+int enqueue_rk3_stages(azb_solver* s, double2* u, double C) {
+  int rc;
+  if ((rc = enqueue_rhs(s, u, nullptr, s->u1, K_RK1, 0, C))) return rc;   // u1 = u + dt L(u)
+  if ((rc = enqueue_rhs(s, s->u1, u, s->u2, K_RK2, 1, C))) return rc;     // u2 = 3/4 u + 1/4 (u1 + dt L(u1))
+  if ((rc = enqueue_rhs(s, s->u2, u, u, K_RK3, 2, C))) return rc;         // u = 2/3 (u2 + dt L(u2)) + 1/3 u
+  return AZB_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* azb_last_error(void) { return g_err.c_str(); }
+
+int azb_create(int dim, int N, int M, int n_vars, int precision, azb_sv_params visc, void* stream,
+               azb_solver** out) {
+  if (!out) return fail(AZB_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  if (dim != 2 && dim != 3) return fail(AZB_ERR_ARG, "dim must be 2 or 3");
+  if (n_vars != dim) return fail(AZB_ERR_UNSUPPORTED, "n_vars must equal dim (no tracer)");
+  if (N < 8 || N % 4 != 0) return fail(AZB_ERR_ARG, "N must be a multiple of 4");
+  if (M == 0) M = 3 * N / 2;
+  if (M != 3 * N / 2) return fail(AZB_ERR_UNSUPPORTED, "only the default padding M = 3N/2 is built");
+  if (M != 96 && M != 192 && M != 384 && M != 768 && M != 1536)
+    return fail(AZB_ERR_UNSUPPORTED, "padded size " + std::to_string(M) + " not built");
+  if (precision != AZB_FP64 && precision != AZB_FP32_MIXED)
+    return fail(AZB_ERR_ARG, "unknown precision mode");
+  azb_solver* s = new azb_solver();
+  s->dim = dim;
+  s->N = N;
+  s->M = M;
+  s->nh = N / 2 + 1;
+  s->nvars = n_vars;
+  s->prec = precision;
+  s->visc = visc;
+  s->tsize = precision == AZB_FP64 ? 8 : 4;
+  const size_t cs = 2 * s->tsize;  // complex at working precision
+  const long nh = s->nh;
+  size_t xs, ys, zs;
+  if (dim == 3) {
+    s->nmodes = (long)N * N * nh;
+    xs = cs * 6 * (size_t)M * N * nh;                                   // T1 (3) / S2 (6)
+    ys = cs * std::max<size_t>(3 * (size_t)M * M * nh, 6 * (size_t)N * N * nh);  // T2 / B_hat
+    zs = cs * 6 * (size_t)M * M * nh;                                   // S1
+  } else {
+    s->nmodes = (long)N * nh;
+    xs = cs * 2 * (size_t)M * nh;
+    ys = cs * 3 * (size_t)N * nh;
+    zs = cs * 3 * (size_t)M * nh;
+  }
+  const size_t us = sizeof(double2) * (size_t)dim * s->nmodes;
+  auto cleanup = [&](int code, const std::string& m) {
+    azb_destroy(s);
+    return fail(code, m);
+  };
+  if (hipMalloc(&s->X, xs) != hipSuccess || hipMalloc(&s->Y, ys) != hipSuccess ||
+      hipMalloc(&s->Z, zs) != hipSuccess || hipMalloc((void**)&s->u1, us) != hipSuccess ||
+      hipMalloc((void**)&s->u2, us) != hipSuccess ||
+      hipMalloc((void**)&s->st, sizeof(DevState)) != hipSuccess)
+    return cleanup(AZB_ERR_ALLOC, "device allocation of the solver workspace failed");
+  s->bytes = xs + ys + zs + 2 * us + sizeof(DevState);
+  int rc = precision == AZB_FP64 ? make_twiddles<double>(M, &s->tw) : make_twiddles<float>(M, &s->tw);
+  if (rc) {
+    std::string m = g_err;
+    return cleanup(rc, m);
+  }
+  s->stream = (hipStream_t)stream;  // NULL: the default (null) stream
+  DevState h{};
+  if (hipMemcpy(s->st, &h, sizeof(DevState), hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup(AZB_ERR_HIP, "hipMemcpy of the device state failed");
+  *out = s;
+  return AZB_OK;
+}
+
+int azb_destroy(azb_solver* s) {
+  if (!s) return AZB_OK;
+  (void)hipStreamSynchronize(s->stream);
+  (void)hipFree(s->X);
+  (void)hipFree(s->Y);
+  (void)hipFree(s->Z);
+  (void)hipFree(s->u1);
+  (void)hipFree(s->u2);
+  (void)hipFree(s->st);
+  (void)hipFree(s->tw);
+  if (s->own_stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+  return AZB_OK;
+}
+
+size_t azb_workspace_bytes(const azb_solver* s) { return s ? s->bytes : 0; }
+void* azb_stream(const azb_solver* s) { return s ? (void*)s->stream : nullptr; }
+
+int azb_dudt(azb_solver* s, const void* u_hat, void* dudt_hat) {
+  if (!s || !u_hat || !dudt_hat) return fail(AZB_ERR_ARG, "NULL argument");
+  hipLaunchKernelGGL(step_prologue_fixed, dim3(1), dim3(256), 0, s->stream, s->st, 0.0);
+  return enqueue_rhs(s, (const double2*)u_hat, nullptr, (double2*)dudt_hat, K_DUDT, 3, 0.0);
+}
+
+int azb_u_max(azb_solver* s, double* u_max_host) {
+  if (!s || !u_max_host) return fail(AZB_ERR_ARG, "NULL argument");
+  HIPCHK(hipMemcpyAsync(u_max_host, &s->st->u_max, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return AZB_OK;
+}
+
+int azb_dt(azb_solver* s, double C, double* dt_host) {
+  double um = 0;
+  int rc = azb_u_max(s, &um);
+  if (rc) return rc;
+  if (!dt_host) return fail(AZB_ERR_ARG, "NULL argument");
+  *dt_host = C / (s->N * um);
+  return AZB_OK;
+}
+
+int azb_rk3_step(azb_solver* s, void* u_hat, double max_dt, double C, double* dt_host) {
+  if (!s || !u_hat) return fail(AZB_ERR_ARG, "NULL argument");
+  hipLaunchKernelGGL(step_prologue_fixed, dim3(1), dim3(256), 0, s->stream, s->st, max_dt);
+  int rc = enqueue_rk3_stages(s, (double2*)u_hat, C);
+  if (rc) return rc;
+  if (dt_host) {
+    HIPCHK(hipMemcpyAsync(dt_host, &s->st->dt, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+  }
+  return AZB_OK;
+}
+
+int azb_simulate_until(azb_solver* s, void* u_hat, double t_end, double C, double* time_inout,
+                       int max_steps, int chunk, int* steps_out) {
+  if (!s || !u_hat || !time_inout) return fail(AZB_ERR_ARG, "NULL argument");
+  if (chunk <= 0) chunk = 8;
+  const double pi = 3.14159265358979323846;
+  DevState h{};
+  h.time = *time_inout;
+  h.t_end = t_end;
+  h.max_dt_visc = C * 2. / (s->visc.eps * ((pi * s->N) * (pi * s->N)));
+  h.done = 0;
+  h.steps = 0;
+  HIPCHK(hipMemcpyAsync(s->st, &h, sizeof(DevState), hipMemcpyHostToDevice, s->stream));
+  int taken = 0;
+  struct {
+    double time;
+    int done, steps;
+  } probe{};
+  while (true) {
+    int n = chunk;
+    if (max_steps > 0 && taken + n > max_steps) n = max_steps - taken;
+    for (int i = 0; i < n; ++i) {
+      hipLaunchKernelGGL(step_prologue_loop, dim3(1), dim3(256), 0, s->stream, s->st);
+      int rc = enqueue_rk3_stages(s, (double2*)u_hat, C);
+      if (rc) return rc;
+      hipLaunchKernelGGL(step_epilogue_loop, dim3(1), dim3(64), 0, s->stream, s->st);
+    }
+    taken += n;
+    HIPCHK(hipMemcpyAsync(&probe.time, &s->st->time, sizeof(double), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipMemcpyAsync(&probe.steps, &s->st->steps, sizeof(int), hipMemcpyDeviceToHost, s->stream));
+    HIPCHK(hipStreamSynchronize(s->stream));
+    if (!(probe.time < t_end)) break;
+    if (max_steps > 0 && taken >= max_steps) break;
+  }
+  if (steps_out) *steps_out = probe.steps;
+  *time_inout = (probe.time < t_end) ? probe.time : t_end;
+  return AZB_OK;
+}
+
+int azb_synchronize(azb_solver* s) {
+  if (!s) return fail(AZB_ERR_ARG, "NULL solver");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return AZB_OK;
+}
+
+int azb_device_synchronize(void) {
+  HIPCHK(hipDeviceSynchronize());
+  return AZB_OK;
+}
+
+int azb_malloc(void** ptr, size_t bytes) {
+  if (!ptr) return fail(AZB_ERR_ARG, "NULL argument");
+  HIPCHK(hipMalloc(ptr, bytes));
+  return AZB_OK;
+}
+int azb_free(void* ptr) {
+  HIPCHK(hipFree(ptr));
+  return AZB_OK;
+}
+int azb_memcpy_h2d(void* dst, const void* src_host, size_t bytes) {
+  HIPCHK(hipMemcpy(dst, src_host, bytes, hipMemcpyHostToDevice));
+  return AZB_OK;
+}
+int azb_memcpy_d2h(void* dst_host, const void* src, size_t bytes) {
+  HIPCHK(hipMemcpy(dst_host, src, bytes, hipMemcpyDeviceToHost));
+  return AZB_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// standalone seams
+// ---------------------------------------------------------------------------
+namespace {
+template <typename T>
+int fft_T(int dim, int n, int howmany, bool forward, void* in, void* out, hipStream_t st) {
+  if (!good_len(n)) return fail(AZB_ERR_UNSUPPORTED, "FFT length " + std::to_string(n) + " not built");
+  if (dim < 1 || dim > 3) return fail(AZB_ERR_ARG, "dim must be 1..3");
+  void* tw = nullptr;
+  int rc = make_twiddles<T>(n, &tw);
+  if (rc) return rc;
+  const long nh = n / 2 + 1;
+  long lines = howmany;
+  for (int d = 1; d < dim; ++d) lines *= n;
+  ColArgs c{};
+  c.nsmall = n;
+  c.tw = tw;
+  DevState* dst = nullptr;
+  HIPCHK(hipMalloc((void**)&dst, sizeof(DevState)));
+  HIPCHK(hipMemsetAsync(dst, 0, sizeof(DevState), st));
+  c.st = dst;
+  PlainRowArgs p{};
+  p.nlines = lines;
+  p.tw = tw;
+  // leading complex axes (axis d-2 .. 0), in place on the complex array
+  auto leading = [&](void* arr, int dir) -> int {
+    for (int ax = dim - 2; ax >= 0; --ax) {
+      long inner = nh;  // product of extents after axis ax
+      for (int d = ax + 1; d < dim - 1; ++d) inner *= n;
+      long batches = howmany;
+      for (int d = 0; d < ax; ++d) batches *= n;
+      c.in = arr;
+      c.out = arr;
+      c.ncols = inner;
+      c.in_rs = c.out_rs = inner;
+      c.in_bs = c.out_bs = inner * n;
+      int r = dir < 0 ? launch_col_full<T, -1>(n, c, batches, st) : launch_col_full<T, +1>(n, c, batches, st);
+      if (r) return r;
+    }
+    return AZB_OK;
+  };
+  if (forward) {
+    p.in = in;
+    p.out = out;
+    if ((rc = launch_plain_row<T, -1>(n, p, st))) return rc;
+    if ((rc = leading(out, -1))) return rc;
+  } else {
+    if ((rc = leading(in, +1))) return rc;
+    p.in = in;
+    p.out = out;
+    if ((rc = launch_plain_row<T, +1>(n, p, st))) return rc;
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(st));
+  (void)hipFree(tw);
+  (void)hipFree(dst);
+  return AZB_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int azb_fft_r2c(int dim, int n, int howmany, int precision, const void* in, void* out, void* stream) {
+  if (!in || !out) return fail(AZB_ERR_ARG, "NULL argument");
+  hipStream_t st = (hipStream_t)stream;
+  return precision == AZB_FP64 ? fft_T<double>(dim, n, howmany, true, (void*)in, out, st)
+                               : fft_T<float>(dim, n, howmany, true, (void*)in, out, st);
+}
+
+int azb_fft_c2r(int dim, int n, int howmany, int precision, void* in, void* out, void* stream) {
+  if (!in || !out) return fail(AZB_ERR_ARG, "NULL argument");
+  hipStream_t st = (hipStream_t)stream;
+  return precision == AZB_FP64 ? fft_T<double>(dim, n, howmany, false, in, out, st)
+                               : fft_T<float>(dim, n, howmany, false, in, out, st);
+}
+
+int azb_copy_to_padded(int dim, int N, int M, const void* in, void* out, void* stream) {
+  if (!in || !out) return fail(AZB_ERR_ARG, "NULL argument");
+  if ((dim != 2 && dim != 3) || M < N) return fail(AZB_ERR_ARG, "bad dim or sizes");
+  const long mh = M / 2 + 1;
+  const long total = dim == 2 ? (long)M * mh : (long)M * M * mh;
+  hipLaunchKernelGGL(pad_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, (const double2*)in, (double2*)out, dim, N, M);
+  HIPCHK(hipGetLastError());
+  return AZB_OK;
+}
+
+int azb_leray(int dim, int N, void* u_hat, void* stream) {
+  if (!u_hat) return fail(AZB_ERR_ARG, "NULL argument");
+  if (dim != 2 && dim != 3) return fail(AZB_ERR_ARG, "dim must be 2 or 3");
+  const long nh = N / 2 + 1;
+  const long nmodes = dim == 2 ? (long)N * nh : (long)N * N * nh;
+  hipLaunchKernelGGL(leray_kernel, dim3((unsigned)((nmodes + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, (double2*)u_hat, dim, N, nmodes);
+  HIPCHK(hipGetLastError());
+  return AZB_OK;
+}
+
+int azb_energy_spectrum(int dim, int N, const void* u_hat, double* spec_host, void* stream) {
+  if (!u_hat || !spec_host) return fail(AZB_ERR_ARG, "NULL argument");
+  if (dim != 2 && dim != 3) return fail(AZB_ERR_ARG, "dim must be 2 or 3");
+  hipStream_t st = (hipStream_t)stream;
+  const long nh = N / 2 + 1;
+  const long nmodes = dim == 2 ? (long)N * nh : (long)N * N * nh;
+  double* d = nullptr;
+  HIPCHK(hipMalloc((void**)&d, sizeof(double) * nh));
+  HIPCHK(hipMemsetAsync(d, 0, sizeof(double) * nh, st));
+  long blocks = (nmodes + 255) / 256;
+  if (blocks > 2048) blocks = 2048;
+  hipLaunchKernelGGL(energy_spectrum_kernel, dim3((unsigned)blocks), dim3(256), sizeof(double) * nh, st,
+                     (const double2*)u_hat, dim, N, nmodes, d);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(spec_host, d, sizeof(double) * nh, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  HIPCHK(hipFree(d));
+  return AZB_OK;
+}
+
+int azb_axpby(double a, const void* x, double b, void* y, long n, void* stream) {
+  if (!x || !y) return fail(AZB_ERR_ARG, "NULL argument");
+  if (n <= 0) return AZB_OK;
+  hipLaunchKernelGGL(axpby_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, a,
+                     (const double2*)x, b, (double2*)y, n);
+  HIPCHK(hipGetLastError());
+  return AZB_OK;
+}
+
+int azb_max_norm(const void* x, long n, double* out_host, void* stream) {
+  if (!x || !out_host) return fail(AZB_ERR_ARG, "NULL argument");
+  hipStream_t st = (hipStream_t)stream;
+  unsigned long long* d = nullptr;
+  HIPCHK(hipMalloc((void**)&d, sizeof(unsigned long long)));
+  HIPCHK(hipMemsetAsync(d, 0, sizeof(unsigned long long), st));
+  if (n > 0) {
+    long blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(max_abs_kernel, dim3((unsigned)blocks), dim3(256), 0, st, (const double*)x, n, d);
+  }
+  unsigned long long bits = 0;
+  HIPCHK(hipMemcpyAsync(&bits, d, sizeof(bits), hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  (void)hipFree(d);
+  double v;
+  std::memcpy(&v, &bits, sizeof(v));
+  *out_host = v;
+  return AZB_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,272 @@
+"""Python mirror of azeban's Equation / TimeIntegrator / Simulation API for the
+IncompressibleEuler<Dim, SmoothCutoff1D, NoForcing> + SSP_RK3 path, running on
+the HIP library through its C ABI.  Device arrays are torch tensors (complex128
+on the GPU); torch is used for device memory only.
+
+Reference interfaces mirrored:
+  Grid<Dim>                         include/azeban/grid.hpp:30-53
+  SmoothCutoff1D / factory          include/azeban/equations/spectral_viscosity.hpp:56-77,
+                                    include/azeban/equations/spectral_viscosity_factory.hpp:29-54
+  IncompressibleEuler::dudt, dt     include/azeban/equations/incompressible_euler.hpp:103-124
+  SSP_RK3::integrate                include/azeban/evolution/ssp_rk3.hpp:48-64
+  Simulation::simulate_until        src/azeban/simulation.cpp:89-102
+  FFT forward/backward              include/azeban/operations/fft.hpp:37-121
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+from dataclasses import dataclass
+
+import torch
+
+from . import _capi
+from ._capi import AZB_FP32_MIXED, AZB_FP64, SVParams, check, load
+
+PRECISIONS = {"fp64": AZB_FP64, "fp32_mixed": AZB_FP32_MIXED}
+
+
+def _ptr(t: torch.Tensor) -> C.c_void_p:
+    if not t.is_cuda:
+        raise ValueError("expected a device (cuda/hip) tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream() -> C.c_void_p:
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+@dataclass
+class Grid:
+    """Grid<Dim>: N_phys, N_phys_pad (default 3N/2), N_fourier = N/2+1."""
+
+    dim: int
+    N_phys: int
+    N_phys_pad: int = 0
+
+    def __post_init__(self):
+        if self.N_phys_pad == 0:
+            self.N_phys_pad = 3 * self.N_phys // 2
+        self.N_fourier = self.N_phys // 2 + 1
+        self.N_fourier_pad = self.N_phys_pad // 2 + 1
+
+    def shape_fourier(self, n_vars: int):
+        return (n_vars,) + (self.N_phys,) * (self.dim - 1) + (self.N_fourier,)
+
+    def shape_phys(self, n_vars: int):
+        return (n_vars,) + (self.N_phys,) * self.dim
+
+
+@dataclass
+class SmoothCutoff1D:
+    eps: float
+    s: float
+    k0: float
+
+
+def make_smooth_cutoff_1d(config: dict, grid: Grid) -> SmoothCutoff1D:
+    """Factory of spectral_viscosity_factory.hpp:29-54 (missing "eps" is an error)."""
+    s = float(config.get("s", 1.0))
+    theta = float(config.get("theta", (2 * s - 1) / (2 * s)))
+    m0 = float(config.get("m0", 2 * math.pi))
+    if "eps" not in config:
+        raise ValueError('Smooth Cutoff Viscosity expects "eps" parameter in config')
+    eps = float(config["eps"])
+    return SmoothCutoff1D(eps / float(grid.N_phys) ** (2 * s - 1), s, m0 * float(grid.N_phys) ** theta)
+
+
+class IncompressibleEuler:
+    """Equation<Dim> for incompressible Euler with SmoothCutoff1D viscosity, no forcing.
+
+    Owns the HIP solver handle (workspace, stage buffers, FFT twiddles).
+    """
+
+    def __init__(self, grid: Grid, visc: SmoothCutoff1D, precision: str = "fp64"):
+        self.grid = grid
+        self.visc_ = visc
+        self.precision = precision
+        lib = load()
+        h = C.c_void_p()
+        check(lib.azb_create(grid.dim, grid.N_phys, grid.N_phys_pad, grid.dim, PRECISIONS[precision],
+                             SVParams(visc.eps, visc.s, visc.k0), _stream(), C.byref(h)))
+        self.handle = h
+
+    def __del__(self):
+        h = getattr(self, "handle", None)
+        if h:
+            load().azb_destroy(h)
+            self.handle = None
+
+    def n_vars(self) -> int:
+        return self.grid.dim
+
+    def visc(self) -> float:
+        return self.visc_.eps
+
+    def workspace_bytes(self) -> int:
+        return int(load().azb_workspace_bytes(self.handle))
+
+    def dudt(self, dudt_hat: torch.Tensor, u_hat: torch.Tensor, t: float = 0.0, dt: float = 0.0,
+             C_: float = 0.0) -> None:
+        """dudt_hat <- L(u_hat); dudt_hat may be u_hat itself."""
+        self._check(u_hat)
+        self._check(dudt_hat)
+        check(load().azb_dudt(self.handle, _ptr(u_hat), _ptr(dudt_hat)))
+
+    def dt(self, C_: float) -> float:
+        out = C.c_double()
+        check(load().azb_dt(self.handle, C_, C.byref(out)))
+        return out.value
+
+    def u_max(self) -> float:
+        out = C.c_double()
+        check(load().azb_u_max(self.handle, C.byref(out)))
+        return out.value
+
+    def _check(self, u: torch.Tensor):
+        if tuple(u.shape) != self.grid.shape_fourier(self.n_vars()) or u.dtype != torch.complex128:
+            raise ValueError(f"expected complex128 of shape {self.grid.shape_fourier(self.n_vars())}, "
+                             f"got {u.dtype} {tuple(u.shape)}")
+
+
+class SSP_RK3:
+    """TimeIntegrator<Dim>: Shu-Osher SSP-RK3 with dt from the first stage's CFL."""
+
+    def __init__(self, shape, equation: IncompressibleEuler):
+        self.shape = tuple(shape)
+        self.equation_ = equation
+
+    def equation(self) -> IncompressibleEuler:
+        return self.equation_
+
+    def integrate(self, t: float, max_dt: float, C_: float, u: torch.Tensor) -> float:
+        self.equation_._check(u)
+        out = C.c_double()
+        check(load().azb_rk3_step(self.equation_.handle, _ptr(u), max_dt, C_, C.byref(out)))
+        return out.value
+
+
+class Simulation:
+    """Simulation<Dim>: owns u_hat, runs the time loop to a target time."""
+
+    def __init__(self, grid: Grid, C_: float, timestepper: SSP_RK3, u_hat: torch.Tensor | None = None):
+        self.grid = grid
+        self.C = C_
+        self.timestepper = timestepper
+        n = timestepper.equation().n_vars()
+        if u_hat is None:
+            self.u_ = torch.zeros(grid.shape_fourier(n), dtype=torch.complex128, device="cuda")
+        else:
+            self.u_ = u_hat.detach().to(device="cuda", dtype=torch.complex128).clone().contiguous()
+        self.time_ = 0.0
+        self.steps = 0
+
+    def u(self) -> torch.Tensor:
+        return self.u_
+
+    def time(self) -> float:
+        return self.time_
+
+    def set_time(self, t: float):
+        self.time_ = t
+
+    def reset(self):
+        self.time_ = 0.0
+
+    def simulate_until(self, t: float, max_steps: int = 0, chunk: int = 8) -> int:
+        """Device-driven loop (dt stays on the device); returns the number of steps."""
+        eq = self.timestepper.equation()
+        time = C.c_double(self.time_)
+        steps = C.c_int(0)
+        check(load().azb_simulate_until(eq.handle, _ptr(self.u_), t, self.C, C.byref(time), max_steps,
+                                        chunk, C.byref(steps)))
+        self.time_ = time.value
+        self.steps += steps.value
+        return steps.value
+
+    def simulate_for(self, t: float) -> int:
+        return self.simulate_until(self.time_ + t)
+
+    def simulate_until_host(self, t: float) -> int:
+        """Host-driven loop, step by step exactly as Simulation::simulate_until does."""
+        eq = self.timestepper.equation()
+        n = 0
+        while self.time_ < t:
+            max_dt = min(t - self.time_, self.C * 2.0 / (eq.visc() * (math.pi * self.grid.N_phys) ** 2))
+            dt = self.timestepper.integrate(t, max_dt, self.C, self.u_)
+            self.time_ += dt
+            n += 1
+        self.time_ = t
+        self.steps += n
+        return n
+
+
+# ---------------------------------------------------------------------------
+# functional seams
+# ---------------------------------------------------------------------------
+def fft_r2c(u: torch.Tensor) -> torch.Tensor:
+    """Unnormalised R2C over all but the leading (batch) axis."""
+    dim = u.dim() - 1
+    n = u.shape[-1]
+    prec = AZB_FP64 if u.dtype == torch.float64 else AZB_FP32_MIXED
+    cdt = torch.complex128 if u.dtype == torch.float64 else torch.complex64
+    out = torch.empty(tuple(u.shape[:-1]) + (n // 2 + 1,), dtype=cdt, device=u.device)
+    u = u.contiguous()
+    check(load().azb_fft_r2c(dim, n, u.shape[0], prec, _ptr(u), _ptr(out), _stream()))
+    return out
+
+
+def fft_c2r(u_hat: torch.Tensor, n: int) -> torch.Tensor:
+    """Unnormalised C2R; works on a copy (the library overwrites its input)."""
+    dim = u_hat.dim() - 1
+    prec = AZB_FP64 if u_hat.dtype == torch.complex128 else AZB_FP32_MIXED
+    rdt = torch.float64 if u_hat.dtype == torch.complex128 else torch.float32
+    tmp = u_hat.contiguous().clone()
+    out = torch.empty(tuple(u_hat.shape[:-1]) + (n,), dtype=rdt, device=u_hat.device)
+    check(load().azb_fft_c2r(dim, n, u_hat.shape[0], prec, _ptr(tmp), _ptr(out), _stream()))
+    return out
+
+
+def leray_(u_hat: torch.Tensor) -> torch.Tensor:
+    dim = u_hat.dim() - 1
+    check(load().azb_leray(dim, u_hat.shape[1], _ptr(u_hat), _stream()))
+    return u_hat
+
+
+def initialize(u_phys) -> torch.Tensor:
+    """Initializer::initialize: u_hat = FFT(u) (unnormalised) then Leray, on the device."""
+    u = torch.as_tensor(u_phys, dtype=torch.float64).to("cuda").contiguous()
+    return leray_(fft_r2c(u))
+
+
+def energy_spectrum(u_hat: torch.Tensor):
+    dim = u_hat.dim() - 1
+    N = u_hat.shape[1]
+    spec = (C.c_double * (N // 2 + 1))()
+    check(load().azb_energy_spectrum(dim, N, _ptr(u_hat), spec, _stream()))
+    return list(spec)
+
+
+def copy_to_padded(src: torch.Tensor, M: int) -> torch.Tensor:
+    """One component, N -> M zero padding."""
+    dim = src.dim()
+    N = src.shape[0]
+    shape = (M,) * (dim - 1) + (M // 2 + 1,)
+    out = torch.empty(shape, dtype=torch.complex128, device=src.device)
+    src = src.contiguous()
+    check(load().azb_copy_to_padded(dim, N, M, _ptr(src), _ptr(out), _stream()))
+    return out
+
+
+def axpby_(a: float, x: torch.Tensor, b: float, y: torch.Tensor) -> torch.Tensor:
+    check(load().azb_axpby(a, _ptr(x), b, _ptr(y), x.numel(), _stream()))
+    return y
+
+
+def max_norm(x: torch.Tensor) -> float:
+    out = C.c_double()
+    x = x.contiguous()
+    check(load().azb_max_norm(_ptr(x), x.numel(), C.byref(out), _stream()))
+    return out.value

@@ -1,0 +1,141 @@
+/*
+ * azeban_hip.h -- C ABI of the MI355X (gfx950) spectral-viscosity Euler RHS and
+ * SSP-RK3 time stepper.
+ *
+ * The entry points replace the device seams of TobiasRohner/azeban's
+ * IncompressibleEuler<Dim, SmoothCutoff1D, NoForcing>::dudt + SSP_RK3::integrate
+ * path.  Reference interfaces each one stands in for are cited per declaration
+ * (paths relative to the reference repository root).
+ *
+ * Conventions
+ *  - All arrays are device pointers unless the name ends in _host.
+ *  - Fourier state u_hat: interleaved complex double (double2 / azeban complex_t),
+ *    shape (n_vars, N, ..., N/2+1), row-major, components outermost
+ *    (include/azeban/grid.hpp:126-133).
+ *  - FFTs are unnormalised; forward is e^{-ikx}, backward e^{+ikx}
+ *    (src/azeban/operations/fftwfft.cpp:321-350).
+ *  - Every function returns AZB_OK (0) or a negative error code; the message is
+ *    available from azb_last_error().  Calls are asynchronous on the solver's
+ *    stream unless documented as synchronising.
+ */
+#ifndef AZEBAN_HIP_H_
+#define AZEBAN_HIP_H_
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AZB_OK 0
+#define AZB_ERR_ARG -1
+#define AZB_ERR_HIP -2
+#define AZB_ERR_UNSUPPORTED -3
+#define AZB_ERR_ALLOC -4
+
+/* precision modes */
+#define AZB_FP64 0        /* everything in FP64 (the drop-in path) */
+#define AZB_FP32_MIXED 1  /* FFTs and products in FP32; RK state, reductions, SV weights FP64 */
+
+typedef struct azb_solver azb_solver;
+
+/* SmoothCutoff1D{eps, s, k0}: nu(k) = -eps k^{2s} (1 - exp(-(|k|/k0)^18))
+ * (include/azeban/equations/spectral_viscosity.hpp:56-77). */
+typedef struct {
+  double eps;
+  double s;
+  double k0;
+} azb_sv_params;
+
+/* Error string of the last failing call on this thread. */
+const char* azb_last_error(void);
+
+/* Build an equation + integrator for a dim-D grid of N points per axis padded to
+ * M = 3N/2 (the default of include/azeban/grid.hpp:44-49 and
+ * include/azeban/grid_factory.hpp:28-92).  Allocates the device workspace and
+ * the RK3 stage buffers; this stands in for the IncompressibleEuler constructor
+ * (include/azeban/equations/incompressible_euler.hpp:56-96) plus the SSP_RK3
+ * constructor (include/azeban/evolution/ssp_rk3.hpp:36-39).  stream is the
+ * hipStream_t all of the solver's work is queued on; NULL selects the default
+ * (null) stream. */
+int azb_create(int dim, int N, int M, int n_vars, int precision, azb_sv_params visc,
+               void* stream, azb_solver** out);
+int azb_destroy(azb_solver* s);
+
+/* Bytes of device memory held by the solver (workspace + stage buffers + twiddles). */
+size_t azb_workspace_bytes(const azb_solver* s);
+/* The solver's HIP stream (hipStream_t). */
+void* azb_stream(const azb_solver* s);
+
+/* dudt_hat = L(u_hat): one RHS evaluation, IncompressibleEuler::dudt
+ * (include/azeban/equations/incompressible_euler.hpp:103-120).  dudt may alias
+ * u_hat.  Also records max|u| / M^d for azb_dt. */
+int azb_dudt(azb_solver* s, const void* u_hat, void* dudt_hat);
+
+/* C / (N u_max) of the latest RHS evaluation, IncompressibleEuler::dt
+ * (incompressible_euler.hpp:122-124).  Synchronising. */
+int azb_dt(azb_solver* s, double C, double* dt_host);
+/* max|u| / M^d of the latest RHS evaluation.  Synchronising. */
+int azb_u_max(azb_solver* s, double* u_max_host);
+
+/* One SSP-RK3 step in place on u_hat with dt = min(C/(N u_max(stage 1)), max_dt),
+ * SSP_RK3::integrate (include/azeban/evolution/ssp_rk3.hpp:48-64).  dt is taken on
+ * the device; dt_host (if not NULL) receives it (synchronising). */
+int azb_rk3_step(azb_solver* s, void* u_hat, double max_dt, double C, double* dt_host);
+
+/* Simulation::simulate_until (src/azeban/simulation.cpp:89-102): while time < t_end,
+ * take an RK3 step with max_dt = min(t_end - time, C*2/(eps (pi N)^2)).  The loop
+ * runs on the device; the host checks completion every `chunk` steps.  On return
+ * *time_inout = t_end (the reference sets time_ = t after its loop) and *steps_out
+ * holds the number of steps taken.  max_steps > 0 stops early (then *time_inout is
+ * the time reached).  Synchronising. */
+int azb_simulate_until(azb_solver* s, void* u_hat, double t_end, double C, double* time_inout,
+                       int max_steps, int chunk, int* steps_out);
+
+/* ---- standalone operations (test seams) -------------------------------------- */
+
+/* Rank-dim R2C, unnormalised, batch `howmany`, contiguous per batch entry:
+ * in real (howmany, n, .., n) -> out complex (howmany, n, .., n/2+1).
+ * precision selects double or float arrays.  FFTWFFT_R2C/CUFFT_R2C::forward
+ * (src/azeban/operations/fftwfft.cpp:258-262, src/azeban/cuda/operations/cufft.cpp:134-140). */
+int azb_fft_r2c(int dim, int n, int howmany, int precision, const void* in, void* out, void* stream);
+/* Rank-dim C2R, unnormalised; complex passes over the leading axes (in place, the
+ * input is overwritten as FFTW's and cuFFT's c2r may do), then c2r over the last
+ * axis, which ignores Im at k_last = 0 and n/2.  FFTWFFT_R2C/CUFFT_R2C::backward
+ * (fftwfft.cpp:265-269, cufft.cpp:143-149). */
+int azb_fft_c2r(int dim, int n, int howmany, int precision, void* in, void* out, void* stream);
+
+/* One component N -> M zero padding, copy_to_padded
+ * (src/azeban/operations/copy_to_padded.cpp:301-328,
+ *  include/azeban/operations/copy_to_padded_functions.hpp:11-63). */
+int azb_copy_to_padded(int dim, int N, int M, const void* in, void* out, void* stream);
+
+/* y <- a x + b y over n complex doubles, axpby (include/azeban/operations/axpby.hpp:28-56). */
+int azb_axpby(double a, const void* x, double b, void* y, long n, void* stream);
+
+/* max |x| over n doubles, max_norm (include/azeban/operations/norm.hpp:72-91).  Synchronising. */
+int azb_max_norm(const void* x, long n, double* out_host, void* stream);
+
+/* Leray projection in place on u_hat (dim components on the N-grid), k = 0 set to 0
+ * (src/azeban/operations/leray.cpp:28-116; used by Initializer::initialize,
+ * include/azeban/init/initializer.hpp:68-75). */
+int azb_leray(int dim, int N, void* u_hat, void* stream);
+
+/* Energy spectrum E(K), K = 0..N/2, over L-infinity shells of u_hat into spec_host
+ * (N/2+1 doubles), energy_spectrum (src/azeban/operations/energy_spectrum.cpp:10-50,
+ * include/azeban/operations/spectrum.hpp:33-118).  Synchronising. */
+int azb_energy_spectrum(int dim, int N, const void* u_hat, double* spec_host, void* stream);
+
+/* ---- memory helpers (so callers need no HIP headers) ------------------------- */
+int azb_malloc(void** ptr, size_t bytes);
+int azb_free(void* ptr);
+int azb_memcpy_h2d(void* dst, const void* src_host, size_t bytes);
+int azb_memcpy_d2h(void* dst_host, const void* src, size_t bytes);
+int azb_synchronize(azb_solver* s);
+int azb_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AZEBAN_HIP_H_ */

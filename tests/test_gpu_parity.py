@@ -1,0 +1,254 @@
+"""Parity of the HIP path (through the C ABI) against the FP64 CPU oracle.
+
+Bars: FFT, RHS and single steps to round-off (relative 1e-12 or tighter);
+trajectories to relative L2 <= 1e-11 after 100 steps (FP64); FP32-mixed to
+relative L2 <= 1e-5; zero padding, Leray and max-norm known answers exactly.
+"""
+import math
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import azeban_oracle as O
+
+import azeban_amd as A
+from azeban_amd import init as I
+
+pytestmark = pytest.mark.gpu
+
+
+def dev(x):
+    return torch.from_numpy(np.ascontiguousarray(x)).to("cuda")
+
+
+def host(t):
+    torch.cuda.synchronize()
+    return t.detach().cpu().numpy()
+
+
+def rel(a, b):
+    return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
+
+
+def random_state(dim, N, seed=3):
+    print(f"[case] random fBm state dim={dim} N={N} seed={seed}", flush=True)
+    return O.initialize(I.fractional_brownian(dim, N, seed=seed))
+
+
+# ---------------------------------------------------------------------------
+# FFT seams (test/azeban/operations/fft.cpp:219-354 analogue: n = 128, 1-2 vars)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("dim,n,howmany", [(1, 128, 1), (1, 128, 2), (2, 128, 1), (2, 128, 2),
+                                           (3, 128, 1), (3, 128, 2), (1, 96, 3), (1, 768, 2),
+                                           (2, 192, 2), (3, 96, 1), (2, 1536, 1)])
+def test_fft_r2c_c2r_fp64(cuda, dim, n, howmany):
+    print(f"[case] fft dim={dim} n={n} howmany={howmany}", flush=True)
+    rng = np.random.default_rng(7)
+    u = rng.uniform(-1, 1, (howmany,) + (n,) * dim)
+    ref = O.fft_r2c(u)
+    got = host(A.fft_r2c(dev(u)))
+    assert np.max(np.abs(got - ref)) <= 1e-8
+    assert rel(got, ref) <= 1e-14
+    back = host(A.fft_c2r(dev(ref), n))
+    assert np.max(np.abs(back / n**dim - u)) <= 1e-8
+    assert rel(back, O.fft_c2r(ref, n)) <= 1e-14
+
+
+@pytest.mark.parametrize("dim,n", [(1, 128), (2, 128), (3, 96)])
+def test_fft_c2r_non_hermitian_convention(cuda, dim, n):
+    """A half spectrum whose k_last = 0 (and n/2) entries are not Hermitian: the
+    result must equal complex passes on the leading axes, then a c2r that ignores
+    Im at k_last in {0, n/2} (the FFTW/cuFFT convention the oracle follows)."""
+    print(f"[case] c2r non-Hermitian dim={dim} n={n}", flush=True)
+    rng = np.random.default_rng(11)
+    shape = (2,) + (n,) * (dim - 1) + (n // 2 + 1,)
+    X = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    got = host(A.fft_c2r(dev(X), n))
+    assert rel(got, O.fft_c2r(X, n)) <= 1e-14
+
+
+def test_fft_fp32(cuda):
+    print("[case] fft fp32 3D n=96", flush=True)
+    rng = np.random.default_rng(5)
+    u = rng.uniform(-1, 1, (2, 96, 96, 96))
+    got = host(A.fft_r2c(dev(u.astype(np.float32))))
+    assert rel(got, O.fft_r2c(u)) <= 1e-6
+
+
+# ---------------------------------------------------------------------------
+# pad / axpby / max_norm / leray / spectrum
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("dim,N", [(2, 64), (2, 128), (3, 64)])
+def test_copy_to_padded_exact(cuda, dim, N):
+    print(f"[case] pad dim={dim} N={N}", flush=True)
+    M = 3 * N // 2
+    rng = np.random.default_rng(1)
+    shape = (N,) * (dim - 1) + (N // 2 + 1,)
+    src = rng.uniform(-1, 1, shape) + 1j * rng.uniform(-1, 1, shape)
+    got = host(A.copy_to_padded(dev(src), M))
+    ref = O.copy_to_padded(src, (M,) * (dim - 1) + (M // 2 + 1,))
+    assert np.array_equal(got, ref)
+
+
+def test_max_norm_known_answer(cuda):
+    """test/azeban/operations/norm.cpp:25-41: max_norm of a constant array of length n is n."""
+    n = 1000 * 1000
+    x = torch.full((n,), float(n), dtype=torch.float64, device="cuda")
+    assert abs(A.max_norm(x) - n) <= 1e-10
+    y = torch.linspace(-3.0, 2.0, 12345, dtype=torch.float64, device="cuda")
+    assert A.max_norm(y) == 3.0
+
+
+def test_axpby(cuda):
+    rng = np.random.default_rng(2)
+    x = rng.standard_normal(1000) + 1j * rng.standard_normal(1000)
+    y = rng.standard_normal(1000) + 1j * rng.standard_normal(1000)
+    got = host(A.axpby_(0.75, dev(x), 0.25, dev(y)))
+    assert np.allclose(got, 0.75 * x + 0.25 * y, rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("dim,N", [(2, 128), (3, 64)])
+def test_initialize_and_leray(cuda, dim, N):
+    print(f"[case] initialize dim={dim} N={N}", flush=True)
+    u = I.fractional_brownian(dim, N, seed=4)
+    got = host(A.initialize(u))
+    ref = O.initialize(u)
+    assert rel(got, ref) <= 1e-13
+    # divergence at round-off
+    i = np.arange(N)
+    ks = 2 * math.pi * np.where(i >= N // 2 + 1, i - N, i)
+    k = [ks] * (dim - 1) + [2 * math.pi * np.arange(N // 2 + 1)]
+    g = np.meshgrid(*k, indexing="ij")
+    div = sum(g[d] * got[d] for d in range(dim))
+    assert np.max(np.abs(div)) <= 1e-12 * np.max(np.abs(got)) * N
+
+
+@pytest.mark.parametrize("dim,N", [(2, 128), (3, 64)])
+def test_energy_spectrum(cuda, dim, N):
+    uh = random_state(dim, N, seed=9)
+    got = np.array(A.energy_spectrum(dev(uh)))
+    ref = O.energy_spectrum(uh, N)
+    assert np.allclose(got, ref, rtol=1e-12, atol=1e-300)
+
+
+# ---------------------------------------------------------------------------
+# RHS (IncompressibleEuler::dudt)
+# ---------------------------------------------------------------------------
+def make_pair(dim, N, precision="fp64"):
+    g = A.Grid(dim, N)
+    visc = A.SmoothCutoff1D(0.05 / N, 1.0, 2 * math.pi * N / 4)
+    eq = A.IncompressibleEuler(g, visc, precision)
+    oeq = O.IncompressibleEuler(O.Grid(dim, N), O.SmoothCutoff1D(0.05 / N, 1.0, 2 * math.pi * N / 4))
+    return g, eq, oeq
+
+
+@pytest.mark.parametrize("dim,N", [(2, 64), (2, 128), (2, 256), (2, 1024), (3, 64)])
+def test_dudt_fp64(cuda, dim, N):
+    uh = random_state(dim, N)
+    g, eq, oeq = make_pair(dim, N)
+    ref = oeq.dudt(uh)
+    u = dev(uh)
+    out = torch.empty_like(u)
+    eq.dudt(out, u)
+    got = host(out)
+    assert rel(got, ref) <= 1e-12, rel(got, ref)
+    assert abs(eq.u_max() - oeq.u_max) <= 1e-14 * oeq.u_max
+    assert abs(eq.dt(0.2) - oeq.dt(0.2)) <= 1e-14 * oeq.dt(0.2)
+    # k = 0 mode is zero
+    assert np.all(got[(slice(None),) + (0,) * dim] == 0)
+
+
+def test_dudt_in_place(cuda):
+    """dudt(x, x) aliasing contract (test/azeban/equations/incompressible_euler.cpp:194)."""
+    uh = random_state(2, 128)
+    g, eq, oeq = make_pair(2, 128)
+    u = dev(uh)
+    eq.dudt(u, u)
+    assert rel(host(u), oeq.dudt(uh)) <= 1e-12
+
+
+def test_taylor_green_2d_steady(cuda):
+    """2D Taylor-Green is a steady state of inviscid Euler and its modes sit below the
+    viscosity cutoff, so dudt must vanish to round-off at full size (N = 1024)."""
+    N = 1024
+    print("[case] TG2D steady N=1024", flush=True)
+    g, eq, _ = make_pair(2, N)
+    u = A.initialize(I.taylor_green_2d(N))
+    out = torch.empty_like(u)
+    eq.dudt(out, u)
+    assert float(out.abs().max()) <= 1e-11 * float(u.abs().max())
+    assert abs(eq.u_max() - (N / (1.5 * N)) ** 2) <= 1e-13
+
+
+# ---------------------------------------------------------------------------
+# RK3 steps and trajectories
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("dim,N", [(2, 128), (3, 64)])
+def test_rk3_step(cuda, dim, N):
+    uh = random_state(dim, N, seed=5)
+    g, eq, oeq = make_pair(dim, N)
+    rk = A.SSP_RK3(g.shape_fourier(dim), eq)
+    u = dev(uh)
+    dt = rk.integrate(1.0, 1.0, 0.2, u)
+    ref, dt_ref = O.ssp_rk3_integrate(oeq, uh, 1.0, 0.2)
+    assert abs(dt - dt_ref) <= 1e-15 * dt_ref
+    assert rel(host(u), ref) <= 1e-12
+
+
+def test_trajectory_2d_dsl_100_steps(cuda):
+    """Double shear layer at N = 128, 100 steps: FP64 relative L2 <= 1e-11."""
+    N = 128
+    print("[case] DSL N=128 100 steps", flush=True)
+    uh = O.initialize(I.double_shear_layer(N, seed=1))
+    g, eq, oeq = make_pair(2, N)
+    sim = A.Simulation(g, 0.2, A.SSP_RK3(g.shape_fourier(2), eq), dev(uh))
+    osim = O.Simulation(oeq, uh, 0.2)
+    steps = sim.simulate_until(1.0, max_steps=100)
+    osim.simulate_until(1.0, max_steps=100)
+    assert steps == 100
+    assert abs(sim.time() - osim.time) <= 1e-12
+    assert rel(host(sim.u()), osim.u) <= 1e-11
+    assert np.allclose(A.energy_spectrum(sim.u()), O.energy_spectrum(osim.u, N), rtol=1e-10, atol=1e-30)
+
+
+def test_trajectory_3d_tg_20_steps(cuda):
+    N = 64
+    print("[case] TG3D N=64 20 steps", flush=True)
+    uh = O.initialize(I.taylor_green_3d(N))
+    g, eq, oeq = make_pair(3, N)
+    sim = A.Simulation(g, 0.2, A.SSP_RK3(g.shape_fourier(3), eq), dev(uh))
+    osim = O.Simulation(oeq, uh, 0.2)
+    sim.simulate_until(1.0, max_steps=20)
+    osim.simulate_until(1.0, max_steps=20)
+    assert rel(host(sim.u()), osim.u) <= 1e-11
+
+
+def test_simulate_until_device_loop_matches_host_loop(cuda):
+    """The device-driven loop (dt and time on the device) takes the same steps as the
+    host loop of Simulation::simulate_until, including the final clamp onto t."""
+    N = 64
+    uh = O.initialize(I.double_shear_layer(N, seed=2))
+    g, eq, _ = make_pair(2, N)
+    a = A.Simulation(g, 0.2, A.SSP_RK3(g.shape_fourier(2), eq), dev(uh))
+    g2, eq2, _ = make_pair(2, N)
+    b = A.Simulation(g2, 0.2, A.SSP_RK3(g2.shape_fourier(2), eq2), dev(uh))
+    na = a.simulate_until(0.05, chunk=4)
+    nb = b.simulate_until_host(0.05)
+    assert na == nb
+    assert a.time() == b.time() == 0.05
+    assert rel(host(a.u()), host(b.u())) <= 1e-14
+
+
+@pytest.mark.parametrize("dim,N,steps", [(2, 256, 20), (3, 64, 10)])
+def test_fp32_mixed_error(cuda, dim, N, steps):
+    print(f"[case] fp32_mixed dim={dim} N={N}", flush=True)
+    uh = random_state(dim, N, seed=6)
+    g, eq, oeq = make_pair(dim, N, "fp32_mixed")
+    sim = A.Simulation(g, 0.2, A.SSP_RK3(g.shape_fourier(dim), eq), dev(uh))
+    osim = O.Simulation(oeq, uh, 0.2)
+    sim.simulate_until(10.0, max_steps=steps)
+    osim.simulate_until(10.0, max_steps=steps)
+    err = rel(host(sim.u()), osim.u)
+    print(f"fp32_mixed rel L2 vs FP64 oracle after {steps} steps: {err:.3e}")
+    assert err <= 1e-5
