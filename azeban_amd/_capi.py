@@ -13,6 +13,8 @@ LIB_PATH = Path(__file__).resolve().parent / "libazeban_hip.so"
 AZB_FP64 = 0
 AZB_FP32_MIXED = 1
 
+PROF_FAMILIES = ["inv_ax0", "inv_ax1", "row_fused", "fwd_ax1", "fwd_ax0", "kspace_rk"]
+
 
 class AzbError(RuntimeError):
     pass
@@ -53,6 +55,9 @@ SIGNATURES = {
     "azb_free": (_i, [_vp]),
     "azb_memcpy_h2d": (_i, [_vp, _vp, _sz]),
     "azb_memcpy_d2h": (_i, [_vp, _vp, _sz]),
+    "azb_profile_enable": (_i, [_vp, _i]),
+    "azb_profile_read": (_i, [_vp, _dp, C.POINTER(C.c_long), _i]),
+    "azb_profile_reset": (_i, [_vp]),
     "azb_synchronize": (_i, [_vp]),
     "azb_device_synchronize": (_i, []),
 }

@@ -48,6 +48,23 @@ def build(verbose: bool = False) -> Path:
     return LIB
 
 
+CPP_TEST_SRC = ROOT / "tests" / "cpp" / "test_dropin.cpp"
+CPP_TEST_BIN = ROOT / "tests" / "cpp" / "test_dropin"
+
+
+def build_cpp_tests(verbose: bool = False) -> Path:
+    """The C++ drop-in layer test program, host-only C++ linked against the library."""
+    cmd = ["g++", "-std=c++17", "-O2", "-Wall", f"-I{ROOT / 'include'}", str(CPP_TEST_SRC),
+           f"-L{PKG}", "-lazeban_hip", "-Wl,-rpath,$ORIGIN/../../azeban_amd", "-o", str(CPP_TEST_BIN)]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("g++ failed building the C++ drop-in tests")
+    return CPP_TEST_BIN
+
+
 def is_stale() -> bool:
     if not LIB.exists():
         return True

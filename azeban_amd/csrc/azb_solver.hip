@@ -178,9 +178,46 @@ struct azb_solver {
   void* tw = nullptr;
   size_t bytes = 0;
   long nmodes = 0;  // modes per component on the N-grid
+  // per-kernel-family hipEvent timing (off unless enabled)
+  bool prof = false;
+  std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_ev;
+  std::vector<hipEvent_t> ev_pool;
+  double prof_ms[AZB_PROF_FAMILIES] = {};
+  long prof_n[AZB_PROF_FAMILIES] = {};
 };
 
 namespace {
+
+hipEvent_t pool_event(azb_solver* s) {
+  if (!s->ev_pool.empty()) {
+    hipEvent_t e = s->ev_pool.back();
+    s->ev_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+// RAII bracket of one kernel family launch on the solver's stream
+struct ProfScope {
+  azb_solver* s;
+  int fam;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(azb_solver* s_, int fam_) : s(s_), fam(fam_) {
+    if (s->prof) {
+      a = pool_event(s);
+      b = pool_event(s);
+      (void)hipEventRecord(a, s->stream);
+    }
+  }
+  ~ProfScope() {
+    if (s->prof) {
+      (void)hipEventRecord(b, s->stream);
+      s->prof_ev.push_back({fam, {a, b}});
+    }
+  }
+};
 
 template <typename T>
 int enqueue_rhs_T(azb_solver* s, const double2* u_in, const double2* u0, double2* out, int kmode,
@@ -202,39 +239,63 @@ int enqueue_rhs_T(azb_solver* s, const double2* u_in, const double2* u0, double2
     c.in = u_in; c.out = s->X; c.ncols = (long)N * nh;
     c.in_rs = (long)N * nh; c.out_rs = (long)N * nh;
     c.in_bs = (long)N * N * nh; c.out_bs = (long)M * N * nh;
-    if ((rc = launch_col<double, T, T, +1, COL_PAD_IN>(M, c, 3, st))) return rc;
+    {
+      ProfScope p(s, AZB_PROF_INV_AX0);
+      if ((rc = launch_col<double, T, T, +1, COL_PAD_IN>(M, c, 3, st))) return rc;
+    }
     // K2: inverse along axis 1; X[c][i'][j][k] -> Y[c][i'][j'][k]
     c.in = s->X; c.out = s->Y; c.ncols = nh;
     c.in_rs = nh; c.out_rs = nh;
     c.in_bs = (long)N * nh; c.out_bs = (long)M * nh;
-    if ((rc = launch_col<T, T, T, +1, COL_PAD_IN>(M, c, 3L * M, st))) return rc;
+    {
+      ProfScope p(s, AZB_PROF_INV_AX1);
+      if ((rc = launch_col<T, T, T, +1, COL_PAD_IN>(M, c, 3L * M, st))) return rc;
+    }
     // K3: last axis c2r, products, max, r2c; Y[c][line][k] -> Z[m][line][k]
     r.in = s->Y; r.out = s->Z; r.nlines = (long)M * M;
-    if ((rc = launch_row<T, 3>(M, r, st))) return rc;
+    {
+      ProfScope p(s, AZB_PROF_ROW);
+      if ((rc = launch_row<T, 3>(M, r, st))) return rc;
+    }
     // K4: forward along axis 1, M -> N rows; Z[m][i'][j'][k] -> X[m][i'][j][k]
     c.in = s->Z; c.out = s->X; c.ncols = nh;
     c.in_rs = nh; c.out_rs = nh;
     c.in_bs = (long)M * nh; c.out_bs = (long)N * nh;
-    if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 6L * M, st))) return rc;
+    {
+      ProfScope p(s, AZB_PROF_FWD_AX1);
+      if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 6L * M, st))) return rc;
+    }
     // K5: forward along axis 0; X[m][i'][j][k] -> Y[m][i][j][k] (B_hat on the N-grid)
     c.in = s->X; c.out = s->Y; c.ncols = (long)N * nh;
     c.in_rs = (long)N * nh; c.out_rs = (long)N * nh;
     c.in_bs = (long)M * N * nh; c.out_bs = (long)N * N * nh;
-    if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 6, st))) return rc;
+    {
+      ProfScope p(s, AZB_PROF_FWD_AX0);
+      if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 6, st))) return rc;
+    }
   } else {
     // K1: inverse along axis 0; [c][i][k] -> X[c][i'][k]
     c.in = u_in; c.out = s->X; c.ncols = nh;
     c.in_rs = nh; c.out_rs = nh;
     c.in_bs = (long)N * nh; c.out_bs = (long)M * nh;
-    if ((rc = launch_col<double, T, T, +1, COL_PAD_IN>(M, c, 2, st))) return rc;
+    {
+      ProfScope p(s, AZB_PROF_INV_AX0);
+      if ((rc = launch_col<double, T, T, +1, COL_PAD_IN>(M, c, 2, st))) return rc;
+    }
     // K2: last axis; X[c][i'][k] -> Z[m][i'][k]
     r.in = s->X; r.out = s->Z; r.nlines = M;
-    if ((rc = launch_row<T, 2>(M, r, st))) return rc;
+    {
+      ProfScope p(s, AZB_PROF_ROW);
+      if ((rc = launch_row<T, 2>(M, r, st))) return rc;
+    }
     // K3: forward along axis 0; Z[m][i'][k] -> Y[m][i][k]
     c.in = s->Z; c.out = s->Y; c.ncols = nh;
     c.in_rs = nh; c.out_rs = nh;
     c.in_bs = (long)M * nh; c.out_bs = (long)N * nh;
-    if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 3, st))) return rc;
+    {
+      ProfScope p(s, AZB_PROF_FWD_AX0);
+      if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 3, st))) return rc;
+    }
   }
   const double Md = std::pow((double)M, s->dim);
   hipLaunchKernelGGL(finalize_umax, dim3(1), dim3(64), 0, st, s->st, slot, kmode == K_RK1 ? 1 : 0,
@@ -253,10 +314,13 @@ int enqueue_rhs_T(azb_solver* s, const double2* u_in, const double2* u0, double2
   k.mode = kmode;
   k.st = s->st;
   const unsigned blocks = (unsigned)((s->nmodes + 255) / 256);
-  if (s->dim == 3)
-    hipLaunchKernelGGL((kspace_kernel<T, 3>), dim3(blocks), dim3(256), 0, st, k);
-  else
-    hipLaunchKernelGGL((kspace_kernel<T, 2>), dim3(blocks), dim3(256), 0, st, k);
+  {
+    ProfScope p(s, AZB_PROF_KSPACE);
+    if (s->dim == 3)
+      hipLaunchKernelGGL((kspace_kernel<T, 3>), dim3(blocks), dim3(256), 0, st, k);
+    else
+      hipLaunchKernelGGL((kspace_kernel<T, 2>), dim3(blocks), dim3(256), 0, st, k);
+  }
   HIPCHK(hipGetLastError());
   return AZB_OK;
 }
@@ -352,6 +416,11 @@ int azb_destroy(azb_solver* s) {
   (void)hipFree(s->u2);
   (void)hipFree(s->st);
   (void)hipFree(s->tw);
+  for (auto& e : s->prof_ev) {
+    (void)hipEventDestroy(e.second.first);
+    (void)hipEventDestroy(e.second.second);
+  }
+  for (auto e : s->ev_pool) (void)hipEventDestroy(e);
   if (s->own_stream) (void)hipStreamDestroy(s->stream);
   delete s;
   return AZB_OK;
@@ -429,6 +498,46 @@ int azb_simulate_until(azb_solver* s, void* u_hat, double t_end, double C, doubl
   }
   if (steps_out) *steps_out = probe.steps;
   *time_inout = (probe.time < t_end) ? probe.time : t_end;
+  return AZB_OK;
+}
+
+int azb_profile_enable(azb_solver* s, int on) {
+  if (!s) return fail(AZB_ERR_ARG, "NULL solver");
+  s->prof = on != 0;
+  return AZB_OK;
+}
+
+int azb_profile_read(azb_solver* s, double* ms_host, long* launches_host, int n) {
+  if (!s || !ms_host || !launches_host) return fail(AZB_ERR_ARG, "NULL argument");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  for (auto& e : s->prof_ev) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e.second.first, e.second.second));
+    s->prof_ms[e.first] += ms;
+    s->prof_n[e.first] += 1;
+    s->ev_pool.push_back(e.second.first);
+    s->ev_pool.push_back(e.second.second);
+  }
+  s->prof_ev.clear();
+  for (int i = 0; i < n && i < AZB_PROF_FAMILIES; ++i) {
+    ms_host[i] = s->prof_ms[i];
+    launches_host[i] = s->prof_n[i];
+  }
+  return AZB_OK;
+}
+
+int azb_profile_reset(azb_solver* s) {
+  if (!s) return fail(AZB_ERR_ARG, "NULL solver");
+  HIPCHK(hipStreamSynchronize(s->stream));
+  for (auto& e : s->prof_ev) {
+    s->ev_pool.push_back(e.second.first);
+    s->ev_pool.push_back(e.second.second);
+  }
+  s->prof_ev.clear();
+  for (int i = 0; i < AZB_PROF_FAMILIES; ++i) {
+    s->prof_ms[i] = 0;
+    s->prof_n[i] = 0;
+  }
   return AZB_OK;
 }
 

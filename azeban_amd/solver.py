@@ -125,6 +125,20 @@ class IncompressibleEuler:
         check(load().azb_u_max(self.handle, C.byref(out)))
         return out.value
 
+    # per-kernel-family hipEvent timing on the solver's stream
+    def profile_enable(self, on: bool = True) -> None:
+        check(load().azb_profile_enable(self.handle, 1 if on else 0))
+
+    def profile_reset(self) -> None:
+        check(load().azb_profile_reset(self.handle))
+
+    def profile_read(self) -> dict:
+        n = len(_capi.PROF_FAMILIES)
+        ms = (C.c_double * n)()
+        cnt = (C.c_long * n)()
+        check(load().azb_profile_read(self.handle, ms, cnt, n))
+        return {name: (ms[i], cnt[i]) for i, name in enumerate(_capi.PROF_FAMILIES)}
+
     def _check(self, u: torch.Tensor):
         if tuple(u.shape) != self.grid.shape_fourier(self.n_vars()) or u.dtype != torch.complex128:
             raise ValueError(f"expected complex128 of shape {self.grid.shape_fourier(self.n_vars())}, "

@@ -1,0 +1,212 @@
+"""Throughput of the spectral-viscosity Euler SSP-RK3 step on one MI355X.
+
+Default workload: 3D Taylor-Green, N = 512^3 (M = 768 dealiasing grid), FP64 and
+FP32-mixed, C = 0.2, SmoothCutoff1D(eps = 0.05/N, s = 1, k0 = 2 pi N/4).  A "step"
+is one SSP-RK3 step (three RHS evaluations) with dt taken on the device.
+
+Prints one JSON line: value = grid points x RK steps per second (Mpts*RK-steps/s)
+of the FP64 path, the FP32-mixed figures beside it, the roofline of the dominant
+kernel family (algorithmic bytes / hipEvent-timed launch duration vs 8 TB/s),
+and the CPU oracle timed on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mpts·RK-steps/s at 512^3 FP64 vs FP32-mixed; ensemble samples/s; p99 latency"
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def tg3d_device(N: int):
+    """u = (sin x cos y cos z, -cos x sin y cos z, 0), x = 2 pi i / N, built on the GPU."""
+    import torch
+
+    x = 2 * math.pi * torch.arange(N, dtype=torch.float64, device="cuda") / N
+    s, c = torch.sin(x), torch.cos(x)
+    u = torch.empty((3, N, N, N), dtype=torch.float64, device="cuda")
+    u[0] = s[:, None, None] * c[None, :, None] * c[None, None, :]
+    u[1] = -c[:, None, None] * s[None, :, None] * c[None, None, :]
+    u[2] = 0
+    return u
+
+
+def dsl_device(N: int, seed: int = 1):
+    import torch
+    from azeban_amd import init as I
+
+    return torch.from_numpy(I.double_shear_layer(N, seed=seed)).to("cuda")
+
+
+def run_precision(A, dim, N, u_hat0, precision, steps, warmup):
+    import torch
+
+    g = A.Grid(dim, N)
+    visc = A.SmoothCutoff1D(0.05 / N, 1.0, 2 * math.pi * N / 4)
+    eq = A.IncompressibleEuler(g, visc, precision)
+    sim = A.Simulation(g, 0.2, A.SSP_RK3(g.shape_fourier(dim), eq), u_hat0)
+    t_far = 1e9
+    log(f"{precision}: warmup {warmup} steps")
+    if warmup:
+        sim.simulate_until(t_far, max_steps=warmup, chunk=warmup)
+    torch.cuda.synchronize()
+    log(f"{precision}: timing {steps} steps")
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record()
+    n = sim.simulate_until(t_far, max_steps=steps, chunk=steps)
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    gpu_ms = ev0.elapsed_time(ev1)
+    assert n == steps, (n, steps)
+    # second pass with per-family hipEvents for the roofline
+    log(f"{precision}: profiled pass {steps} steps")
+    eq.profile_reset()
+    eq.profile_enable(True)
+    sim.simulate_until(t_far, max_steps=steps, chunk=steps)
+    fam = eq.profile_read()
+    eq.profile_enable(False)
+    return {"eq": eq, "sim": sim, "wall_s": wall, "gpu_ms": gpu_ms, "families": fam}
+
+
+def cpu_baseline(dim: int, budget_s: float):
+    """The FP64 numpy oracle (pocketfft, one thread) on a bounded sample."""
+    import numpy as np
+
+    sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
+    from oracle import azeban_oracle as O
+    from azeban_amd import init as I
+
+    Ns = 128 if dim == 3 else 1024
+    u0 = O.initialize(I.taylor_green_3d(Ns) if dim == 3 else I.double_shear_layer(Ns))
+    eq = O.IncompressibleEuler(O.Grid(dim, Ns), O.baseline_visc(Ns))
+    sim = O.Simulation(eq, u0, 0.2)
+    t0 = time.perf_counter()
+    steps = 0
+    while time.perf_counter() - t0 < budget_s or steps == 0:
+        sim.step(1e9)
+        steps += 1
+    el = time.perf_counter() - t0
+    name = f"3D Taylor-Green {Ns}^3" if dim == 3 else f"2D double shear layer {Ns}^2"
+    return {
+        "value": Ns**dim * steps / el / 1e6,
+        "unit": "Mpts*RK-steps/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": f"{name}, {steps} SSP-RK3 steps in {el:.1f} s, FP64 numpy oracle "
+                  f"(pocketfft, not FFTW), 1 thread",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["tg3d", "dsl2d"], default="tg3d")
+    ap.add_argument("--N", type=int, default=0)
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
+              "(the multi-GPU sharding of this path is a design, not a build)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import azeban_amd as A
+    from azeban_amd import traffic
+
+    dim = 3 if args.workload == "tg3d" else 2
+    N = args.N or (512 if dim == 3 else 1024)
+    log(f"workload {args.workload} N={N}")
+    u_phys = tg3d_device(N) if dim == 3 else dsl_device(N)
+    u_hat0 = A.initialize(u_phys)
+    del u_phys
+    torch.cuda.synchronize()
+
+    res = {"fp64": run_precision(A, dim, N, u_hat0, "fp64", args.steps, args.warmup)}
+    if not args.no_mixed:
+        res["fp32_mixed"] = run_precision(A, dim, N, u_hat0, "fp32_mixed", args.steps, args.warmup)
+
+    pts = float(N) ** dim
+
+    def summarize(r, precision):
+        ms_step = r["gpu_ms"] / args.steps
+        fam_bytes = traffic.family_bytes(dim, N, precision)
+        kern = {}
+        for name, (ms, cnt) in r["families"].items():
+            if cnt == 0:
+                continue
+            avg = ms / cnt
+            gbs = fam_bytes[name] / (avg * 1e-3) / 1e9
+            kern[name] = {"ms_avg": round(avg, 4), "launches": cnt, "GB_per_launch": round(fam_bytes[name] / 1e9, 4),
+                          "GB_s": round(gbs, 1), "frac_hbm": round(gbs / HBM_PEAK_GBS, 4)}
+        return ms_step, kern
+
+    ms64, kern64 = summarize(res["fp64"], "fp64")
+    value = pts * args.steps / (res["fp64"]["gpu_ms"] * 1e-3) / 1e6
+    dom = max(kern64.items(), key=lambda kv: kv[1]["ms_avg"] * kv[1]["launches"])
+    step_gb = traffic.step_bytes(dim, N, "fp64") / 1e9
+    roofline = {
+        "bound": "hbm",
+        "kernel": dom[0],
+        "achieved": dom[1]["GB_s"],
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": dom[1]["frac_hbm"],
+        "traffic": None,
+        "step_algorithmic_GB": round(step_gb, 3),
+        "step_GB_s": round(step_gb / (ms64 * 1e-3), 1),
+    }
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "Mpts*RK-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms64, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": ("3D Taylor-Green" if dim == 3 else "2D double shear layer")
+                   + f" {N}^{dim}, FP64, SSP-RK3 C=0.2, SmoothCutoff eps=0.05/N cutoff N/4",
+                   "N": N, "dim": dim, "M": 3 * N // 2},
+        "roofline": roofline,
+        "kernels_fp64": kern64,
+    }
+    if "fp32_mixed" in res:
+        msm, kernm = summarize(res["fp32_mixed"], "fp32_mixed")
+        vm = pts * args.steps / (res["fp32_mixed"]["gpu_ms"] * 1e-3) / 1e6
+        a = res["fp64"]["sim"].u()
+        b = res["fp32_mixed"]["sim"].u()
+        err = float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(a))
+        out["fp32_mixed"] = {"value": round(vm, 2), "ms_per_step": round(msm, 3),
+                             "speedup_vs_fp64": round(vm / value, 3),
+                             "rel_l2_vs_fp64_gpu": err,
+                             "steps_compared": args.warmup + 2 * args.steps,
+                             "kernels": kernm}
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        out["cpu_baseline"] = cpu_baseline(dim, args.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

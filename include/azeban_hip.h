@@ -92,6 +92,21 @@ int azb_rk3_step(azb_solver* s, void* u_hat, double max_dt, double C, double* dt
 int azb_simulate_until(azb_solver* s, void* u_hat, double t_end, double C, double* time_inout,
                        int max_steps, int chunk, int* steps_out);
 
+/* ---- per-kernel-family device timing (hipEvent pairs on the solver's stream),
+ * the analogue of azeban's ProfileDevice records (include/azeban/profiler.hpp:35-135). */
+#define AZB_PROF_INV_AX0 0  /* inverse complex pass along axis 0, N -> M rows */
+#define AZB_PROF_INV_AX1 1  /* inverse complex pass along axis 1 (3D), N -> M rows */
+#define AZB_PROF_ROW 2      /* last axis: c2r, u_i u_j products, max|u|, r2c */
+#define AZB_PROF_FWD_AX1 3  /* forward complex pass along axis 1 (3D), M -> N rows */
+#define AZB_PROF_FWD_AX0 4  /* forward complex pass along axis 0, M -> N rows */
+#define AZB_PROF_KSPACE 5   /* ik.B_hat, Leray, viscosity, RK stage update */
+#define AZB_PROF_FAMILIES 6
+int azb_profile_enable(azb_solver* s, int on);
+/* accumulated milliseconds and launch counts per family since the last reset;
+ * synchronising */
+int azb_profile_read(azb_solver* s, double* ms_host, long* launches_host, int n);
+int azb_profile_reset(azb_solver* s);
+
 /* ---- standalone operations (test seams) -------------------------------------- */
 
 /* Rank-dim R2C, unnormalised, batch `howmany`, contiguous per batch entry:
