@@ -378,3 +378,210 @@ struct BlockFFT {
 };
 
 }  // namespace azb
+
+namespace azb {
+
+// ---------------------------------------------------------------------------
+// Workgroup Stockham FFT with a per-pass thread <-> butterfly mapping.
+//   MAP_STD : g = t + NT b, s = g % TS, j = g / TS (sequences fastest across threads)
+//   MAP_SEQ : a thread owns one butterfly j of G consecutive sequences at once, so
+//             after the last pass it holds the same positions of all G sequences
+//             (elementwise products across sequences need no exchange), and a first
+//             pass reading HBM is contiguous in j across threads
+//   MAP_PAIR: a thread owns the butterfly pair (j, L/R - j) of one sequence (with
+//             0 paired with L/(2R)), so every output position and its mirror L - pos
+//             sit in the same thread after the last pass
+// Any mapping can be used by any pass: the LDS exchange writes with the mapping of
+// the pass that produced the data and reads with the mapping of the next pass.
+// ---------------------------------------------------------------------------
+enum { MAP_STD = 0, MAP_SEQ = 1, MAP_PAIR = 2 };
+
+template <int MAP, int TS, int NT, int L, int R, int G>
+__device__ __forceinline__ void bmap(int t, int b, int& s, int& j) {
+  if (MAP == MAP_STD) {
+    const int g = t + NT * b;
+    s = g % TS;
+    j = g / TS;
+  } else if (MAP == MAP_SEQ) {
+    const int q = b / G, gg = b % G;
+    const int u = t + NT * q;
+    j = u % (L / R);
+    s = (u / (L / R)) * G + gg;
+  } else {
+    constexpr int H = L / (2 * R);
+    const int q = b / 2, e = b % 2;
+    const int u = t + NT * q;
+    s = u / H;
+    const int p = u % H;
+    j = e == 0 ? p : (p == 0 ? H : L / R - p);
+  }
+}
+
+template <int... Rs>
+struct PlanAt;
+template <int R0, int... Rs>
+struct PlanAt<R0, Rs...> {
+  static constexpr int count = 1 + sizeof...(Rs);
+};
+
+template <typename T, int L, int TS, int NT, int DIR, bool SPLIT, int G, int MAPF, int MAPL>
+struct BlockFFT2 {
+  static constexpr int E = TS * L / NT;
+  static_assert(TS * L % NT == 0, "TS*L must be a multiple of NT");
+
+  template <int P, int NP>
+  static constexpr int map_of() {
+    return P == 0 ? MAPF : (P == NP - 1 ? MAPL : MAP_STD);
+  }
+
+  __device__ __forceinline__ static cx<T> twiddle(const cx<T>* __restrict__ tw, int e) {
+    cx<T> w = tw[e];
+    if (DIR > 0) w.y = -w.y;
+    return w;
+  }
+
+  template <int MAP, int Ns, int R>
+  __device__ __forceinline__ static void butterflies(cx<T>* v, int t_in, const cx<T>* __restrict__ tw_in) {
+    static_assert(E % R == 0, "radix must divide E");
+    // keep this pass's twiddle loads inside the pass: without the opaque copy the
+    // compiler hoists the (data-independent) loads of every pass of every transform
+    // in the kernel to its top and holds them all in registers
+    const cx<T>* tw = tw_in;
+    asm volatile("" : "+s"(tw));
+    int t = t_in;
+    asm volatile("" : "+v"(t));
+#pragma unroll
+    for (int b = 0; b < E / R; ++b) {
+      cx<T>* w = v + b * R;
+      if (Ns > 1) {
+        int s, j;
+        bmap<MAP, TS, NT, L, R, G>(t, b, s, j);
+        const int k = j % Ns;
+        constexpr int step = L / (Ns * R);
+#pragma unroll
+        for (int r = 1; r < R; ++r) w[r] = cmul(w[r], twiddle(tw, r * k * step));
+      }
+      DFT<R, DIR, T>::run(w);
+    }
+  }
+
+  template <int MAPW, int MAPR, int Ns, int R, int Rn>
+  __device__ __forceinline__ static void exchange(cx<T>* v, int t_in, T* lds) {
+    constexpr int comps = SPLIT ? 2 : 1;
+    // opaque copy of the thread index: LDS addresses are recomputed here instead of
+    // being hoisted (they are loop-invariant across transforms) and held live
+    int t = t_in;
+    asm volatile("" : "+v"(t));
+#pragma unroll
+    for (int c = 0; c < comps; ++c) {
+      __syncthreads();
+#pragma unroll
+      for (int b = 0; b < E / R; ++b) {
+        int s, j;
+        bmap<MAPW, TS, NT, L, R, G>(t, b, s, j);
+        const int k = j % Ns;
+        const int base = (j - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+          const int idx = (base + r * Ns) * TS + s;
+          if (SPLIT) {
+            lds[idx] = c == 0 ? v[b * R + r].x : v[b * R + r].y;
+          } else {
+            lds[2 * idx] = v[b * R + r].x;
+            lds[2 * idx + 1] = v[b * R + r].y;
+          }
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int b = 0; b < E / Rn; ++b) {
+        int s, j;
+        bmap<MAPR, TS, NT, L, Rn, G>(t, b, s, j);
+#pragma unroll
+        for (int r = 0; r < Rn; ++r) {
+          const int idx = (j + r * (L / Rn)) * TS + s;
+          if (SPLIT) {
+            if (c == 0)
+              v[b * Rn + r].x = lds[idx];
+            else
+              v[b * Rn + r].y = lds[idx];
+          } else {
+            v[b * Rn + r].x = lds[2 * idx];
+            v[b * Rn + r].y = lds[2 * idx + 1];
+          }
+        }
+      }
+    }
+  }
+
+  template <int P, int NP, int Ns, int R>
+  __device__ __forceinline__ static void passes(cx<T>* v, int t, T*, const cx<T>* tw, Plan<R>) {
+    butterflies<map_of<P, NP>(), Ns, R>(v, t, tw);
+  }
+  template <int P, int NP, int Ns, int R, int Rn, int... Rs>
+  __device__ __forceinline__ static void passes(cx<T>* v, int t, T* lds, const cx<T>* tw, Plan<R, Rn, Rs...>) {
+    butterflies<map_of<P, NP>(), Ns, R>(v, t, tw);
+    exchange<map_of<P, NP>(), map_of<P + 1, NP>(), Ns, R, Rn>(v, t, lds);
+    passes<P + 1, NP, Ns * R, Rn, Rs...>(v, t, lds, tw, Plan<Rn, Rs...>{});
+  }
+
+  // all threads of the workgroup call run (it holds barriers)
+  template <int... Rs>
+  __device__ __forceinline__ static void run(cx<T>* v, int t, T* lds, const cx<T>* tw, Plan<Rs...> p) {
+    static_assert(plan_prod<Rs...>() == L, "plan must multiply to L");
+    passes<0, PlanAt<Rs...>::count, 1, Rs...>(v, t, lds, tw, p);
+  }
+
+  // run() after load_dft(): the first pass's butterflies are already applied
+  template <int R0>
+  __device__ __forceinline__ static void run_rest(cx<T>*, int, T*, const cx<T>*, Plan<R0>) {}
+  template <int R0, int R1, int... Rs>
+  __device__ __forceinline__ static void run_rest(cx<T>* v, int t, T* lds, const cx<T>* tw, Plan<R0, R1, Rs...>) {
+    static_assert(plan_prod<R0, R1, Rs...>() == L, "plan must multiply to L");
+    constexpr int NP = PlanAt<R0, R1, Rs...>::count;
+    exchange<map_of<0, NP>(), map_of<1, NP>(), 1, R0, R1>(v, t, lds);
+    passes<1, NP, R0, R1, Rs...>(v, t, lds, tw, Plan<R1, Rs...>{});
+  }
+
+  // first-pass load fused with the first pass's butterflies (which need no
+  // twiddles), one butterfly group at a time, so only one group's loads are live
+  template <int R0, class Ld>
+  __device__ __forceinline__ static void load_dft(cx<T>* v, int t_in, Ld ld) {
+    int t = t_in;
+    asm volatile("" : "+v"(t));
+#pragma unroll
+    for (int b = 0; b < E / R0; ++b) {
+      int s, j;
+      bmap<MAPF, TS, NT, L, R0, G>(t, b, s, j);
+#pragma unroll
+      for (int r = 0; r < R0; ++r) v[b * R0 + r] = ld(s, j + r * (L / R0));
+      DFT<R0, DIR, T>::run(v + b * R0);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+
+  // first-pass load with the first pass's mapping: ld(s, pos)
+  template <int R0, class Ld>
+  __device__ __forceinline__ static void load(cx<T>* v, int t, Ld ld) {
+#pragma unroll
+    for (int b = 0; b < E / R0; ++b) {
+      int s, j;
+      bmap<MAPF, TS, NT, L, R0, G>(t, b, s, j);
+#pragma unroll
+      for (int r = 0; r < R0; ++r) v[b * R0 + r] = ld(s, j + r * (L / R0));
+    }
+  }
+  // last-pass store with the last pass's mapping: st(s, pos, value)
+  template <int RL, class St>
+  __device__ __forceinline__ static void store(const cx<T>* v, int t, St st) {
+#pragma unroll
+    for (int b = 0; b < E / RL; ++b) {
+      int s, j;
+      bmap<MAPL, TS, NT, L, RL, G>(t, b, s, j);
+#pragma unroll
+      for (int r = 0; r < RL; ++r) st(s, j + r * (L / RL), v[b * RL + r]);
+    }
+  }
+};
+
+}  // namespace azb

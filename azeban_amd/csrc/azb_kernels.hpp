@@ -71,43 +71,59 @@ constexpr int last_radix(Plan<Rs...>) { return plan_last<Rs...>(); }
 // all L rows.  PAD_IN reads only the N nonzero rows of the zero-padded input;
 // TRUNC_OUT writes only the N rows the N-grid keeps.
 // ---------------------------------------------------------------------------
+// Columns are addressed as (batch, q, k): k in [0, nk) is contiguous in memory,
+// q in [0, nq) steps by in_qs / out_qs.  A workgroup owns TS consecutive k of one
+// (batch, q), so with a q stride that is a multiple of the tile width every
+// segment it moves is cache-line aligned.
 struct ColArgs {
   const void* in;
   void* out;
-  long ncols;     // columns per batch
-  int tiles;      // ceil(ncols / TS)
+  long nq;        // q rows per batch
+  int nk;         // valid columns per q row
+  int ktiles;     // ceil(nk / TS)
   int nsmall;     // N (unpadded length of the pruned side)
-  long in_rs, out_rs;   // row strides (elements)
+  long in_qs, out_qs;   // q strides (elements)
+  long in_rs, out_rs;   // row strides along the transformed axis (elements)
   long in_bs, out_bs;   // batch strides (elements)
   const void* tw;       // twiddles for L
   const DevState* st;
 };
+
+// columns per workgroup: 128-byte segments at working precision
+template <typename T, int L>
+constexpr int col_ts() {
+  return (L % 3 == 0) ? ((6144 * (8 / (int)sizeof(T))) / L < 1 ? 1 : (6144 * (8 / (int)sizeof(T))) / L)
+                      : ((4096 * (8 / (int)sizeof(T))) / L < 1 ? 1 : (4096 * (8 / (int)sizeof(T))) / L);
+}
 
 template <typename Tin, typename Tout, typename T, int L, int TS, int DIR, int MODE>
 __global__ __launch_bounds__(TS * L / LenPlan<L>::E) void col_fft_kernel(ColArgs a) {
   using LP = LenPlan<L>;
   constexpr int E = LP::E;
   constexpr int NT = TS * L / E;
-  constexpr bool SPLIT = sizeof(T) == 8;
-  using F = BlockFFT<T, L, TS, NT, DIR, SPLIT>;
+  // exchange in one barrier pair unless the interleaved buffer would exceed 64 KB
+  constexpr bool SPLIT = TS * L * 2 * (int)sizeof(T) > 65536;
+  using F = BlockFFT2<T, L, TS, NT, DIR, SPLIT, 1, MAP_STD, MAP_STD>;
   constexpr int R0 = first_radix(typename LP::P{});
   constexpr int RL = last_radix(typename LP::P{});
   __shared__ T lds[ExchangeSize<T, L, TS, SPLIT>::words];
   if (a.st->done) return;
 
   const int t = threadIdx.x;
-  const int tile = blockIdx.x % a.tiles;
-  const long batch = blockIdx.x / a.tiles;
-  const long c0 = (long)tile * TS;
+  const int kt = blockIdx.x % a.ktiles;
+  const long rest = blockIdx.x / a.ktiles;
+  const long q = rest % a.nq;
+  const long batch = rest / a.nq;
+  const long c0 = (long)kt * TS;
   const int N = a.nsmall;
   const int half = N / 2;
-  const cx<Tin>* __restrict__ in = (const cx<Tin>*)a.in + batch * a.in_bs + c0;
-  cx<Tout>* __restrict__ out = (cx<Tout>*)a.out + batch * a.out_bs + c0;
-  const long ncols_left = a.ncols - c0;
+  const cx<Tin>* __restrict__ in = (const cx<Tin>*)a.in + batch * a.in_bs + q * a.in_qs + c0;
+  cx<Tout>* __restrict__ out = (cx<Tout>*)a.out + batch * a.out_bs + q * a.out_qs + c0;
+  const long ncols_left = a.nk - c0;
   const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
 
   cx<T> v[E];
-  F::template load<R0>(v, t, [&](int s, int pos) -> cx<T> {
+  F::template load_dft<R0>(v, t, [&](int s, int pos) -> cx<T> {
     long row = pos;
     if (MODE == COL_PAD_IN) {
       if (pos <= half)
@@ -124,7 +140,7 @@ __global__ __launch_bounds__(TS * L / LenPlan<L>::E) void col_fft_kernel(ColArgs
     }
     return r;
   });
-  F::run(v, t, lds, tw, true, typename LP::P{});
+  F::run_rest(v, t, lds, tw, typename LP::P{});
   F::template store<RL>(v, t, [&](int s, int pos, cx<T> val) {
     long row = pos;
     if (MODE == COL_TRUNC_OUT) {
@@ -153,9 +169,10 @@ __global__ __launch_bounds__(TS * L / LenPlan<L>::E) void col_fft_kernel(ColArgs
 // separated into the two half spectra; only k <= N/2 is written.
 // ---------------------------------------------------------------------------
 struct RowArgs {
-  const void* in;   // [c][line][Nh]
-  void* out;        // [m][line][Nh]
+  const void* in;   // [c][line][ls]: N/2+1 modes per line
+  void* out;        // [m][line][ls]
   long nlines;      // lines per component (M for 2D, M*M for 3D)
+  long ls;          // line stride (elements, >= N/2+1)
   double norm;      // 1 / (N^{d/2} M^{d/2})
   const void* tw;
   DevState* st;
@@ -216,7 +233,7 @@ __global__ __launch_bounds__((RowCfg<T, L, P, DIM>::NT)) void euler_row_kernel(R
       const int cl = e / NH;
       const int l = cl % (2 * P);
       const int c = cl / (2 * P);
-      stg[e] = in[((long)c * a.nlines + line0 + l) * NH + k];
+      stg[e] = in[((long)c * a.nlines + line0 + l) * a.ls + k];
     }
   }
   __syncthreads();
@@ -343,8 +360,8 @@ __global__ __launch_bounds__((RowCfg<T, L, P, DIM>::NT)) void euler_row_kernel(R
         const int pair = s / 3, type = s % 3;
         const int m = round * 3 + type;
         const long la = line0 + 2 * pair, lb = la + 1;
-        out[((long)m * a.nlines + la) * NH + k] = mk(ra[q], ia[q]);
-        out[((long)m * a.nlines + lb) * NH + k] = mk(rb[q], ib[q]);
+        out[((long)m * a.nlines + la) * a.ls + k] = mk(ra[q], ia[q]);
+        out[((long)m * a.nlines + lb) * a.ls + k] = mk(rb[q], ib[q]);
       }
     }
   }
@@ -371,6 +388,8 @@ struct KArgs {
   double2* out;
   long nmodes;           // modes per component
   int n, nh;
+  int nhp;               // last-axis stride of B_hat (>= nh)
+  long bstride;          // component stride of B_hat
   double eps, s2, k0;    // SmoothCutoff1D with s2 = 2s
   int mode;
   const DevState* st;
@@ -401,12 +420,13 @@ __global__ __launch_bounds__(256) void kspace_kernel(KArgs a) {
   const double twopi = 2.0 * 3.14159265358979323846;
   const int N = a.n, NH = a.nh;
   const cx<T>* bh = (const cx<T>*)a.bh;
-  auto B = [&](int m) -> double2 {
-    const cx<T> z = bh[(long)m * a.nmodes + idx];
-    return make_double2((double)z.x, (double)z.y);
-  };
   const int kk = (int)(idx % NH);
   long rest = idx / NH;
+  const long bidx = rest * a.nhp + kk;
+  auto B = [&](int m) -> double2 {
+    const cx<T> z = bh[(long)m * a.bstride + bidx];
+    return make_double2((double)z.x, (double)z.y);
+  };
   double absk2;
   double2 L[3];
   if (DIM == 2) {
@@ -744,6 +764,168 @@ __global__ void energy_spectrum_kernel(const double2* u, int dim, int N, long nm
   __syncthreads();
   for (int e = threadIdx.x; e < nh; e += blockDim.x)
     if (sh[e] != 0.0) atomicAdd(&spec[e], sh[e]);
+}
+
+}  // namespace azb
+
+namespace azb {
+
+// ---------------------------------------------------------------------------
+// 3D row kernel, register-resident products.  Per workgroup: P line pairs
+// (a, b), TS = 3P packed sequences, NT = P*L/8 threads, 24 complex per thread.
+// Each of two rounds runs
+//   inverse  : packed c2r of (u1a + i u2a, u3a + i u3b, u1b + i u2b), first pass
+//              loaded straight from HBM, last pass MAP_SEQ so every thread holds
+//              u1, u2, u3 of both lines at the same 8 positions;
+//   products : round 0 -> (B11, B21, B22), round 1 -> (B31, B32, B33), packed
+//              (line a) + i (line b), in registers, feeding the forward FFT's
+//              MAP_SEQ first pass at exactly those positions;
+//   forward  : last pass MAP_PAIR, so Z[k] and Z[L-k] are in one thread: the two
+//              half spectra are separated in registers and k <= N/2 is stored.
+// The inverse is recomputed in round 1 instead of keeping u in LDS, which leaves
+// LDS to the FFT exchange alone (TS*L words).
+// ---------------------------------------------------------------------------
+template <int L>
+struct Row3Plan;
+template <> struct Row3Plan<768> { using Inv = Plan<8, 12, 8>; using Fwd = Plan<8, 8, 12>; static constexpr int INV_MAPF = MAP_SEQ; };
+template <> struct Row3Plan<384> { using Inv = Plan<8, 6, 8>; using Fwd = Plan<8, 4, 12>; static constexpr int INV_MAPF = MAP_SEQ; };
+template <> struct Row3Plan<192> { using Inv = Plan<8, 3, 8>; using Fwd = Plan<8, 2, 12>; static constexpr int INV_MAPF = MAP_SEQ; };
+template <> struct Row3Plan<96> { using Inv = Plan<12, 8>; using Fwd = Plan<8, 12>; static constexpr int INV_MAPF = MAP_STD; };
+
+template <typename T, int L, int P>
+struct Row3Cfg {
+  static constexpr int TS = 3 * P;
+  static constexpr int NT = P * L / 8;
+  static constexpr int E = 24;
+  static constexpr int N = 2 * L / 3;
+  static constexpr int NH = N / 2 + 1;
+  // FP64 is register-bound to two workgroups per CU, whose interleaved exchange
+  // buffers (2 x 74 KB at L = 768) still fit in LDS: one barrier pair per exchange
+  static constexpr bool SPLIT = false;
+  static constexpr int XWORDS = SPLIT ? TS * L : 2 * TS * L;
+};
+
+template <typename T, int L, int P>
+__global__ __launch_bounds__(P * L / 8, (sizeof(T) == 8 ? 2 : 4)) void euler_row3_kernel(RowArgs a) {
+  using C = Row3Cfg<T, L, P>;
+  using PL = Row3Plan<L>;
+  constexpr int NT = C::NT, E = C::E, N = C::N, NH = C::NH, TS = C::TS;
+  using FI = BlockFFT2<T, L, TS, NT, +1, C::SPLIT, 3, PL::INV_MAPF, MAP_SEQ>;
+  using FF = BlockFFT2<T, L, TS, NT, -1, C::SPLIT, 3, MAP_SEQ, MAP_PAIR>;
+  constexpr int RI0 = first_radix(typename PL::Inv{});
+  static_assert(last_radix(typename PL::Inv{}) == 8 && first_radix(typename PL::Fwd{}) == 8,
+                "products need the inverse's last and the forward's first pass on radix 8");
+  constexpr int RL = last_radix(typename PL::Fwd{});
+  __shared__ T X[C::XWORDS];
+  __shared__ double red[NT / 64 + 1];
+  if (a.st->done) return;
+
+  const int t = threadIdx.x;
+  const long line0 = (long)blockIdx.x * (2 * P);
+  const cx<T>* __restrict__ in = (const cx<T>*)a.in;
+  cx<T>* __restrict__ out = (cx<T>*)a.out;
+  const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
+  const T nrm = T(a.norm);
+  double lmax = 0.0;
+
+  // packed inverse input: sequence s of pair s/3, type s%3
+  auto ld_inv = [&](int s, int pos) -> cx<T> {
+    const int pair = s / 3, type = s % 3;
+    const long la = line0 + 2 * pair, lb = la + 1;
+    int kk;
+    int mode;  // 0: zero, 1: direct, 2: mirrored, 3: k = 0
+    if (pos == 0) {
+      kk = 0; mode = 3;
+    } else if (pos <= N / 2) {
+      kk = pos; mode = 1;
+    } else if (pos >= L - N / 2) {
+      kk = L - pos; mode = 2;
+    } else {
+      return mk(T(0), T(0));
+    }
+    long ix, iy;
+    if (type == 0) {
+      ix = (0 * a.nlines + la) * a.ls; iy = (1 * a.nlines + la) * a.ls;
+    } else if (type == 1) {
+      ix = (2 * a.nlines + la) * a.ls; iy = (2 * a.nlines + lb) * a.ls;
+    } else {
+      ix = (0 * a.nlines + lb) * a.ls; iy = (1 * a.nlines + lb) * a.ls;
+    }
+    const cx<T> X0 = in[ix + kk], Y0 = in[iy + kk];
+    if (mode == 1) return mk(X0.x - Y0.y, X0.y + Y0.x);
+    if (mode == 2) return mk(X0.x + Y0.y, Y0.x - X0.y);
+    return mk(X0.x, Y0.x);
+  };
+
+  cx<T> v[E];
+#pragma unroll 1
+  for (int round = 0; round < 2; ++round) {
+    FI::template load_dft<RI0>(v, t, ld_inv);
+    FI::run_rest(v, t, X, tw, typename PL::Inv{});
+    // v[g*8 + r]: packed sequence type g of this thread's pair, same 8 positions
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const cx<T> p0 = v[r], p1 = v[8 + r], p2 = v[16 + r];
+      // p0 = (u1a, u2a), p1 = (u3a, u3b), p2 = (u1b, u2b)
+      if (round == 0) {
+        lmax = fmax(lmax, (double)fmax(fmax(fabs(p0.x), fabs(p0.y)), fmax(fabs(p2.x), fabs(p2.y))));
+        const T a1 = nrm * p0.x, a2 = nrm * p0.y, b1 = nrm * p2.x, b2 = nrm * p2.y;
+        v[r] = mk(a1 * a1, b1 * b1);       // B11
+        v[8 + r] = mk(a2 * a1, b2 * b1);   // B21
+        v[16 + r] = mk(a2 * a2, b2 * b2);  // B22
+      } else {
+        lmax = fmax(lmax, (double)fmax(fabs(p1.x), fabs(p1.y)));
+        const T a1 = nrm * p0.x, a2 = nrm * p0.y, b1 = nrm * p2.x, b2 = nrm * p2.y;
+        const T a3 = nrm * p1.x, b3 = nrm * p1.y;
+        v[r] = mk(a3 * a1, b3 * b1);       // B31
+        v[8 + r] = mk(a3 * a2, b3 * b2);   // B32
+        v[16 + r] = mk(a3 * a3, b3 * b3);  // B33
+      }
+    }
+    FF::run(v, t, X, tw, typename PL::Fwd{});
+    // separation: one MAP_PAIR unit per thread: butterflies j0 = p, j1 = mirror
+    {
+      constexpr int H = L / (2 * RL);
+      int tt = t;
+      asm volatile("" : "+v"(tt));
+      const int s = tt / H, p = tt % H;
+      const int pair = s / 3, m = round * 3 + s % 3;
+      const long la = line0 + 2 * pair, lb = la + 1;
+      cx<T>* oa = out + ((long)m * a.nlines + la) * a.ls;
+      cx<T>* ob = out + ((long)m * a.nlines + lb) * a.ls;
+      // register indices must be compile-time: the p == 0 unit (butterflies 0 and
+      // H, each its own mirror) and the general unit are unrolled separately
+      auto emit = [&](int pos, const cx<T>& zk, const cx<T>& zm) {
+        if (pos <= N / 2) {
+          oa[pos] = mk(T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y));
+          ob[pos] = mk(T(0.5) * (zk.y + zm.y), T(-0.5) * (zk.x - zm.x));
+        }
+      };
+      if (p != 0) {
+#pragma unroll
+        for (int r = 0; r < RL; ++r) {
+          emit(p + (L / RL) * r, v[r], v[RL + RL - 1 - r]);
+          emit(L / RL - p + (L / RL) * r, v[RL + r], v[RL - 1 - r]);
+        }
+      } else {
+#pragma unroll
+        for (int r = 0; r < RL; ++r) {
+          emit((L / RL) * r, v[r], v[(RL - r) % RL]);
+          emit(H + (L / RL) * r, v[RL + r], v[RL + RL - 1 - r]);
+        }
+      }
+    }
+  }
+
+  lmax = wave_max<T>(lmax);
+  if ((t & 63) == 0) red[t >> 6] = lmax;
+  __syncthreads();
+  if (t == 0) {
+    double mx = 0.0;
+    for (int w = 0; w < (NT + 63) / 64; ++w) mx = fmax(mx, red[w]);
+    atomicMax(&a.st->umax_bins[a.slot][blockIdx.x & (kBins - 1)],
+              (unsigned long long)__double_as_longlong(mx));
+  }
 }
 
 }  // namespace azb

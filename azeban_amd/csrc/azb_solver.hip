@@ -40,13 +40,6 @@ bool good_len(int L) {
   }
 }
 
-// columns per workgroup of the column kernels: 6144 complex values per block
-int col_ts(int L) {
-  int ts = 6144 / L;
-  if (L % 3 != 0) ts = 4096 / L;
-  return ts < 1 ? 1 : ts;
-}
-
 // line pairs per workgroup of the fused row kernel
 int row_p(int L) {
   int p = 1536 / L;
@@ -76,9 +69,9 @@ int make_twiddles(int L, void** out) {
 template <typename Tin, typename Tout, typename T, int L, int DIR, int MODE>
 void launch_col_L(ColArgs a, long batches, hipStream_t st) {
   constexpr int E = LenPlan<L>::E;
-  constexpr int TS = (L % 3 == 0) ? (6144 / L < 1 ? 1 : 6144 / L) : (4096 / L < 1 ? 1 : 4096 / L);
-  a.tiles = (int)((a.ncols + TS - 1) / TS);
-  const long blocks = batches * a.tiles;
+  constexpr int TS = azb::col_ts<T, L>();
+  a.ktiles = (a.nk + TS - 1) / TS;
+  const long blocks = batches * a.nq * a.ktiles;
   hipLaunchKernelGGL((col_fft_kernel<Tin, Tout, T, L, TS, DIR, MODE>), dim3((unsigned)blocks),
                      dim3(TS * L / E), 0, st, a);
 }
@@ -120,10 +113,27 @@ void launch_row_L(RowArgs a, hipStream_t st) {
   hipLaunchKernelGGL((euler_row_kernel<T, L, P, DIM>), dim3((unsigned)blocks), dim3(C::NT), 0, st, a);
 }
 
+template <typename T, int L>
+void launch_row3_L(RowArgs a, hipStream_t st) {
+  constexpr int P = 1536 / L;
+  // every workgroup owns exactly P whole line pairs (checked by the caller)
+  const long blocks = a.nlines / (2 * P);
+  hipLaunchKernelGGL((euler_row3_kernel<T, L, P>), dim3((unsigned)blocks), dim3(P * L / 8), 0, st, a);
+}
+
 template <typename T, int DIM>
 int launch_row(int L, RowArgs a, hipStream_t st) {
   const int P = row_p(L);
   if (a.nlines % (2 * P) != 0) return fail(AZB_ERR_UNSUPPORTED, "line count not a multiple of the line-pair tile");
+  if (DIM == 3) {
+    switch (L) {
+      case 96: launch_row3_L<T, 96>(a, st); return AZB_OK;
+      case 192: launch_row3_L<T, 192>(a, st); return AZB_OK;
+      case 384: launch_row3_L<T, 384>(a, st); return AZB_OK;
+      case 768: launch_row3_L<T, 768>(a, st); return AZB_OK;
+      default: break;
+    }
+  }
   switch (L) {
 #define AZB_CASE(LL) \
   case LL: launch_row_L<T, LL, DIM>(a, st); break;
@@ -165,6 +175,7 @@ int launch_plain_row(int L, PlainRowArgs a, hipStream_t st) {
 // ---------------------------------------------------------------------------
 struct azb_solver {
   int dim = 0, N = 0, M = 0, nh = 0, nvars = 0, prec = 0;
+  long nhp = 0;  // last-axis stride of the intermediates: N/2+1 rounded up to 16
   azb_sv_params visc{};
   hipStream_t stream = nullptr;
   bool own_stream = false;
@@ -223,75 +234,84 @@ template <typename T>
 int enqueue_rhs_T(azb_solver* s, const double2* u_in, const double2* u0, double2* out, int kmode,
                   int slot, double C) {
   const int N = s->N, M = s->M, nh = s->nh;
+  const long hp = s->nhp;  // padded last-axis stride of every intermediate
   hipStream_t st = s->stream;
   int rc;
   ColArgs c{};
   c.nsmall = N;
   c.tw = s->tw;
   c.st = s->st;
+  c.nk = nh;
   RowArgs r{};
   r.tw = s->tw;
   r.st = s->st;
   r.slot = slot;
+  r.ls = hp;
   r.norm = 1.0 / (std::pow(std::sqrt((double)N), s->dim) * std::pow(std::sqrt((double)M), s->dim));
   if (s->dim == 3) {
-    // K1: inverse along axis 0, N -> M rows; [c][i][j][k] -> X[c][i'][j][k]
-    c.in = u_in; c.out = s->X; c.ncols = (long)N * nh;
-    c.in_rs = (long)N * nh; c.out_rs = (long)N * nh;
-    c.in_bs = (long)N * N * nh; c.out_bs = (long)M * N * nh;
+    // K1: inverse along axis 0, N -> M rows; state [c][i][j][k] -> X[c][i'][j][hp]
+    c.in = u_in; c.out = s->X; c.nq = N;
+    c.in_qs = nh; c.out_qs = hp;
+    c.in_rs = (long)N * nh; c.out_rs = (long)N * hp;
+    c.in_bs = (long)N * N * nh; c.out_bs = (long)M * N * hp;
     {
       ProfScope p(s, AZB_PROF_INV_AX0);
       if ((rc = launch_col<double, T, T, +1, COL_PAD_IN>(M, c, 3, st))) return rc;
     }
-    // K2: inverse along axis 1; X[c][i'][j][k] -> Y[c][i'][j'][k]
-    c.in = s->X; c.out = s->Y; c.ncols = nh;
-    c.in_rs = nh; c.out_rs = nh;
-    c.in_bs = (long)N * nh; c.out_bs = (long)M * nh;
+    // K2: inverse along axis 1; X[c][i'][j][hp] -> Y[c][i'][j'][hp]
+    c.in = s->X; c.out = s->Y; c.nq = 1;
+    c.in_qs = 0; c.out_qs = 0;
+    c.in_rs = hp; c.out_rs = hp;
+    c.in_bs = (long)N * hp; c.out_bs = (long)M * hp;
     {
       ProfScope p(s, AZB_PROF_INV_AX1);
       if ((rc = launch_col<T, T, T, +1, COL_PAD_IN>(M, c, 3L * M, st))) return rc;
     }
-    // K3: last axis c2r, products, max, r2c; Y[c][line][k] -> Z[m][line][k]
+    // K3: last axis c2r, products, max, r2c; Y[c][line][hp] -> Z[m][line][hp]
     r.in = s->Y; r.out = s->Z; r.nlines = (long)M * M;
     {
       ProfScope p(s, AZB_PROF_ROW);
       if ((rc = launch_row<T, 3>(M, r, st))) return rc;
     }
-    // K4: forward along axis 1, M -> N rows; Z[m][i'][j'][k] -> X[m][i'][j][k]
-    c.in = s->Z; c.out = s->X; c.ncols = nh;
-    c.in_rs = nh; c.out_rs = nh;
-    c.in_bs = (long)M * nh; c.out_bs = (long)N * nh;
+    // K4: forward along axis 1, M -> N rows; Z[m][i'][j'][hp] -> X[m][i'][j][hp]
+    c.in = s->Z; c.out = s->X; c.nq = 1;
+    c.in_qs = 0; c.out_qs = 0;
+    c.in_rs = hp; c.out_rs = hp;
+    c.in_bs = (long)M * hp; c.out_bs = (long)N * hp;
     {
       ProfScope p(s, AZB_PROF_FWD_AX1);
       if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 6L * M, st))) return rc;
     }
-    // K5: forward along axis 0; X[m][i'][j][k] -> Y[m][i][j][k] (B_hat on the N-grid)
-    c.in = s->X; c.out = s->Y; c.ncols = (long)N * nh;
-    c.in_rs = (long)N * nh; c.out_rs = (long)N * nh;
-    c.in_bs = (long)M * N * nh; c.out_bs = (long)N * N * nh;
+    // K5: forward along axis 0; X[m][i'][j][hp] -> Y[m][i][j][hp] (B_hat on the N-grid)
+    c.in = s->X; c.out = s->Y; c.nq = N;
+    c.in_qs = hp; c.out_qs = hp;
+    c.in_rs = (long)N * hp; c.out_rs = (long)N * hp;
+    c.in_bs = (long)M * N * hp; c.out_bs = (long)N * N * hp;
     {
       ProfScope p(s, AZB_PROF_FWD_AX0);
       if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 6, st))) return rc;
     }
   } else {
-    // K1: inverse along axis 0; [c][i][k] -> X[c][i'][k]
-    c.in = u_in; c.out = s->X; c.ncols = nh;
-    c.in_rs = nh; c.out_rs = nh;
-    c.in_bs = (long)N * nh; c.out_bs = (long)M * nh;
+    // K1: inverse along axis 0; state [c][i][k] -> X[c][i'][hp]
+    c.in = u_in; c.out = s->X; c.nq = 1;
+    c.in_qs = 0; c.out_qs = 0;
+    c.in_rs = nh; c.out_rs = hp;
+    c.in_bs = (long)N * nh; c.out_bs = (long)M * hp;
     {
       ProfScope p(s, AZB_PROF_INV_AX0);
       if ((rc = launch_col<double, T, T, +1, COL_PAD_IN>(M, c, 2, st))) return rc;
     }
-    // K2: last axis; X[c][i'][k] -> Z[m][i'][k]
+    // K2: last axis; X[c][i'][hp] -> Z[m][i'][hp]
     r.in = s->X; r.out = s->Z; r.nlines = M;
     {
       ProfScope p(s, AZB_PROF_ROW);
       if ((rc = launch_row<T, 2>(M, r, st))) return rc;
     }
-    // K3: forward along axis 0; Z[m][i'][k] -> Y[m][i][k]
-    c.in = s->Z; c.out = s->Y; c.ncols = nh;
-    c.in_rs = nh; c.out_rs = nh;
-    c.in_bs = (long)M * nh; c.out_bs = (long)N * nh;
+    // K3: forward along axis 0; Z[m][i'][hp] -> Y[m][i][hp]
+    c.in = s->Z; c.out = s->Y; c.nq = 1;
+    c.in_qs = 0; c.out_qs = 0;
+    c.in_rs = hp; c.out_rs = hp;
+    c.in_bs = (long)M * hp; c.out_bs = (long)N * hp;
     {
       ProfScope p(s, AZB_PROF_FWD_AX0);
       if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 3, st))) return rc;
@@ -308,6 +328,8 @@ int enqueue_rhs_T(azb_solver* s, const double2* u_in, const double2* u0, double2
   k.nmodes = s->nmodes;
   k.n = N;
   k.nh = nh;
+  k.nhp = (int)hp;
+  k.bstride = s->dim == 3 ? (long)N * N * hp : (long)N * hp;
   k.eps = s->visc.eps;
   k.s2 = 2 * s->visc.s;
   k.k0 = s->visc.k0;
@@ -370,17 +392,19 @@ int azb_create(int dim, int N, int M, int n_vars, int precision, azb_sv_params v
   s->tsize = precision == AZB_FP64 ? 8 : 4;
   const size_t cs = 2 * s->tsize;  // complex at working precision
   const long nh = s->nh;
+  s->nhp = (nh + 15) / 16 * 16;
+  const long hp = s->nhp;
   size_t xs, ys, zs;
   if (dim == 3) {
     s->nmodes = (long)N * N * nh;
-    xs = cs * 6 * (size_t)M * N * nh;                                   // T1 (3) / S2 (6)
-    ys = cs * std::max<size_t>(3 * (size_t)M * M * nh, 6 * (size_t)N * N * nh);  // T2 / B_hat
-    zs = cs * 6 * (size_t)M * M * nh;                                   // S1
+    xs = cs * 6 * (size_t)M * N * hp;                                   // T1 (3) / S2 (6)
+    ys = cs * std::max<size_t>(3 * (size_t)M * M * hp, 6 * (size_t)N * N * hp);  // T2 / B_hat
+    zs = cs * 6 * (size_t)M * M * hp;                                   // S1
   } else {
     s->nmodes = (long)N * nh;
-    xs = cs * 2 * (size_t)M * nh;
-    ys = cs * 3 * (size_t)N * nh;
-    zs = cs * 3 * (size_t)M * nh;
+    xs = cs * 2 * (size_t)M * hp;
+    ys = cs * 3 * (size_t)N * hp;
+    zs = cs * 3 * (size_t)M * hp;
   }
   const size_t us = sizeof(double2) * (size_t)dim * s->nmodes;
   auto cleanup = [&](int code, const std::string& m) {
@@ -605,7 +629,9 @@ int fft_T(int dim, int n, int howmany, bool forward, void* in, void* out, hipStr
       for (int d = 0; d < ax; ++d) batches *= n;
       c.in = arr;
       c.out = arr;
-      c.ncols = inner;
+      c.nq = 1;
+      c.nk = (int)inner;
+      c.in_qs = c.out_qs = 0;
       c.in_rs = c.out_rs = inner;
       c.in_bs = c.out_bs = inner * n;
       int r = dir < 0 ? launch_col_full<T, -1>(n, c, batches, st) : launch_col_full<T, +1>(n, c, batches, st);
