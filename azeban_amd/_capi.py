@@ -36,7 +36,24 @@ _ip = C.POINTER(C.c_int)
 SIGNATURES = {
     "azb_last_error": (C.c_char_p, []),
     "azb_create": (_i, [_i, _i, _i, _i, _i, SVParams, _vp, C.POINTER(_vp)]),
+    "azb_create_batch": (_i, [_i, _i, _i, _i, _i, SVParams, _i, _vp, C.POINTER(_vp)]),
     "azb_destroy": (_i, [_vp]),
+    "azb_num_samples": (_i, [_vp]),
+    "azb_bytes_estimate": (_sz, [_i, _i, _i, _i]),
+    "azb_loop_begin": (_i, [_vp, _d, _d, _dp]),
+    "azb_enqueue_steps": (_i, [_vp, _vp, _i]),
+    "azb_loop_read": (_i, [_vp, _dp, _ip]),
+    "azb_loop_probe": (_i, [_vp]),
+    "azb_loop_probe_result": (_i, [_vp, _ip, _dp, _ip]),
+    "azb_memcpy_h2d_async": (_i, [_vp, _vp, _sz, _vp]),
+    "azb_memcpy_d2h_async": (_i, [_vp, _vp, _sz, _vp]),
+    "azb_capture_begin": (_i, [_vp]),
+    "azb_capture_end": (_i, [_vp, C.POINTER(_vp)]),
+    "azb_graph_launch": (_i, [_vp, _vp]),
+    "azb_graph_destroy": (_i, [_vp]),
+    "azb_stream_create": (_i, [C.POINTER(_vp)]),
+    "azb_stream_destroy": (_i, [_vp]),
+    "azb_stream_synchronize": (_i, [_vp]),
     "azb_workspace_bytes": (_sz, [_vp]),
     "azb_stream": (_vp, [_vp]),
     "azb_dudt": (_i, [_vp, _vp, _vp]),

@@ -424,10 +424,27 @@ struct PlanAt<R0, Rs...> {
   static constexpr int count = 1 + sizeof...(Rs);
 };
 
-template <typename T, int L, int TS, int NT, int DIR, bool SPLIT, int G, int MAPF, int MAPL>
+// LDS layout of the exchange.  SWZ = false: element (s, pos) at pos*TS + s.
+// SWZ = true: sequence-major with one pad slot per 256-byte bank row and a
+// sequence stride of 1 modulo the row, so lane strides of 1, TS or R*TS in
+// either index stay spread over the banks.
+template <typename T, int L, int TS, bool SWZ>
+struct ExLayout {
+  static constexpr int PADE = 256 / (2 * (int)sizeof(T));
+  static constexpr int LP = ((L + L / PADE + PADE - 1) / PADE) * PADE + 1;
+  static constexpr int elems = SWZ ? TS * LP : TS * L;
+  __device__ __forceinline__ static int at(int s, int pos) {
+    return SWZ ? s * LP + pos + pos / PADE : pos * TS + s;
+  }
+};
+
+template <typename T, int L, int TS, int NT, int DIR, bool SPLIT, int G, int MAPF, int MAPL, bool SWZ = false>
 struct BlockFFT2 {
   static constexpr int E = TS * L / NT;
   static_assert(TS * L % NT == 0, "TS*L must be a multiple of NT");
+  using LY = ExLayout<T, L, TS, SWZ>;
+  // LDS words (of T) the exchange needs
+  static constexpr int lds_words = SPLIT ? LY::elems : 2 * LY::elems;
 
   template <int P, int NP>
   static constexpr int map_of() {
@@ -483,7 +500,7 @@ struct BlockFFT2 {
         const int base = (j - k) * R + k;
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const int idx = (base + r * Ns) * TS + s;
+          const int idx = LY::at(s, base + r * Ns);
           if (SPLIT) {
             lds[idx] = c == 0 ? v[b * R + r].x : v[b * R + r].y;
           } else {
@@ -499,7 +516,7 @@ struct BlockFFT2 {
         bmap<MAPR, TS, NT, L, Rn, G>(t, b, s, j);
 #pragma unroll
         for (int r = 0; r < Rn; ++r) {
-          const int idx = (j + r * (L / Rn)) * TS + s;
+          const int idx = LY::at(s, j + r * (L / Rn));
           if (SPLIT) {
             if (c == 0)
               v[b * Rn + r].x = lds[idx];

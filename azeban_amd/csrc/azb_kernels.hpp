@@ -85,8 +85,9 @@ struct ColArgs {
   long in_qs, out_qs;   // q strides (elements)
   long in_rs, out_rs;   // row strides along the transformed axis (elements)
   long in_bs, out_bs;   // batch strides (elements)
+  int bps;              // batches per sample (samples are stacked batch-major)
   const void* tw;       // twiddles for L
-  const DevState* st;
+  const DevState* st;   // one per sample
 };
 
 // columns per workgroup: 128-byte segments at working precision
@@ -107,13 +108,12 @@ __global__ __launch_bounds__(TS * L / LenPlan<L>::E) void col_fft_kernel(ColArgs
   constexpr int R0 = first_radix(typename LP::P{});
   constexpr int RL = last_radix(typename LP::P{});
   __shared__ T lds[ExchangeSize<T, L, TS, SPLIT>::words];
-  if (a.st->done) return;
-
   const int t = threadIdx.x;
   const int kt = blockIdx.x % a.ktiles;
   const long rest = blockIdx.x / a.ktiles;
   const long q = rest % a.nq;
   const long batch = rest / a.nq;
+  if (a.st[batch / a.bps].done) return;
   const long c0 = (long)kt * TS;
   const int N = a.nsmall;
   const int half = N / 2;
@@ -173,6 +173,7 @@ struct RowArgs {
   void* out;        // [m][line][ls]
   long nlines;      // lines per component (M for 2D, M*M for 3D)
   long ls;          // line stride (elements, >= N/2+1)
+  long in_ss, out_ss;  // sample strides (elements)
   double norm;      // 1 / (N^{d/2} M^{d/2})
   const void* tw;
   DevState* st;
@@ -215,12 +216,15 @@ __global__ __launch_bounds__((RowCfg<T, L, P, DIM>::NT)) void euler_row_kernel(R
   __shared__ cx<T> U[C::TSI * L];
   __shared__ T X[C::XWORDS];
   __shared__ double red[NT / 64 + 1];
-  if (a.st->done) return;
+  const long bps = a.nlines / (2 * P);  // workgroups per sample
+  const long sample = blockIdx.x / bps;
+  DevState* st = a.st + sample;
+  if (st->done) return;
 
   const int t = threadIdx.x;
-  const long line0 = (long)blockIdx.x * (2 * P);
-  const cx<T>* __restrict__ in = (const cx<T>*)a.in;
-  cx<T>* __restrict__ out = (cx<T>*)a.out;
+  const long line0 = (blockIdx.x % bps) * (2 * P);
+  const cx<T>* __restrict__ in = (const cx<T>*)a.in + sample * a.in_ss;
+  cx<T>* __restrict__ out = (cx<T>*)a.out + sample * a.out_ss;
   const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
   const T nrm = T(a.norm);
 
@@ -373,7 +377,7 @@ __global__ __launch_bounds__((RowCfg<T, L, P, DIM>::NT)) void euler_row_kernel(R
   if (t == 0) {
     double m = 0.0;
     for (int w = 0; w < (NT + 63) / 64; ++w) m = fmax(m, red[w]);
-    atomicMax(&a.st->umax_bins[a.slot][blockIdx.x & (kBins - 1)],
+    atomicMax(&st->umax_bins[a.slot][blockIdx.x & (kBins - 1)],
               (unsigned long long)__double_as_longlong(m));
   }
 }
@@ -392,7 +396,7 @@ struct KArgs {
   long bstride;          // component stride of B_hat
   double eps, s2, k0;    // SmoothCutoff1D with s2 = 2s
   int mode;
-  const DevState* st;
+  const DevState* st;    // one per sample
 };
 
 __device__ __forceinline__ double sv_eval(double k, double eps, double s2, double k0) {
@@ -414,12 +418,16 @@ __device__ __forceinline__ double2 cmuli(double k, double2 b) {
 
 template <typename T, int DIM>
 __global__ __launch_bounds__(256) void kspace_kernel(KArgs a) {
-  if (a.st->done) return;
+  // grid.y = sample; samples are stacked with strides DIM*nmodes (state) and
+  // 3(DIM-1)*bstride (B_hat)
+  const long sample = blockIdx.y;
+  const DevState* st = a.st + sample;
+  if (st->done) return;
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= a.nmodes) return;
   const double twopi = 2.0 * 3.14159265358979323846;
   const int N = a.n, NH = a.nh;
-  const cx<T>* bh = (const cx<T>*)a.bh;
+  const cx<T>* bh = (const cx<T>*)a.bh + sample * (3 * (DIM - 1)) * a.bstride;
   const int kk = (int)(idx % NH);
   long rest = idx / NH;
   const long bidx = rest * a.nhp + kk;
@@ -479,10 +487,10 @@ __global__ __launch_bounds__(256) void kspace_kernel(KArgs a) {
     }
   }
   const double nu = sv_eval(sqrt(absk2), a.eps, a.s2, a.k0);
-  const double dt = a.st->dt;
+  const double dt = st->dt;
 #pragma unroll
   for (int c = 0; c < DIM; ++c) {
-    const long o = (long)c * a.nmodes + idx;
+    const long o = (sample * DIM + c) * a.nmodes + idx;
     const double2 u = a.u_in[o];
     double2 d = absk2 == 0 ? make_double2(0.0, 0.0)
                            : make_double2(-L[c].x + nu * u.x, -L[c].y + nu * u.y);
@@ -508,7 +516,8 @@ __global__ __launch_bounds__(256) void kspace_kernel(KArgs a) {
 // Small control kernels (single workgroup)
 // ---------------------------------------------------------------------------
 // clear the per-slot max bins; fixed-step mode sets max_dt from the host value
-__global__ void step_prologue_fixed(DevState* st, double max_dt) {
+__global__ void step_prologue_fixed(DevState* st_all, double max_dt) {
+  DevState* st = st_all + blockIdx.x;
   const int t = threadIdx.x;
   for (int e = t; e < 4 * kBins; e += blockDim.x) (&st->umax_bins[0][0])[e] = 0ull;
   if (t == 0) {
@@ -518,7 +527,8 @@ __global__ void step_prologue_fixed(DevState* st, double max_dt) {
 }
 
 // device-driven loop: a step past t_end does nothing (done = 1)
-__global__ void step_prologue_loop(DevState* st) {
+__global__ void step_prologue_loop(DevState* st_all) {
+  DevState* st = st_all + blockIdx.x;
   const int t = threadIdx.x;
   const bool go = st->time < st->t_end;
   for (int e = t; e < 4 * kBins; e += blockDim.x) (&st->umax_bins[0][0])[e] = 0ull;
@@ -528,7 +538,8 @@ __global__ void step_prologue_loop(DevState* st) {
   }
 }
 
-__global__ void step_epilogue_loop(DevState* st) {
+__global__ void step_epilogue_loop(DevState* st_all) {
+  DevState* st = st_all + blockIdx.x;
   if (threadIdx.x == 0 && !st->done) {
     st->time += st->dt;
     st->steps += 1;
@@ -536,7 +547,8 @@ __global__ void step_epilogue_loop(DevState* st) {
 }
 
 // reduce one slot's bins: u_max = max / M^d; stage 1 also sets dt = min(C/(N u_max), max_dt)
-__global__ void finalize_umax(DevState* st, int slot, int set_dt, double Md, double C, double N) {
+__global__ void finalize_umax(DevState* st_all, int slot, int set_dt, double Md, double C, double N) {
+  DevState* st = st_all + blockIdx.x;
   if (st->done) return;
   const int t = threadIdx.x;  // 64 threads
   double v = __longlong_as_double((long long)st->umax_bins[slot][t]);
@@ -802,7 +814,7 @@ struct Row3Cfg {
   // FP64 is register-bound to two workgroups per CU, whose interleaved exchange
   // buffers (2 x 74 KB at L = 768) still fit in LDS: one barrier pair per exchange
   static constexpr bool SPLIT = false;
-  static constexpr int XWORDS = SPLIT ? TS * L : 2 * TS * L;
+  static constexpr int XWORDS = 2 * ExLayout<T, L, TS, true>::elems;
 };
 
 template <typename T, int L, int P>
@@ -810,20 +822,24 @@ __global__ __launch_bounds__(P * L / 8, (sizeof(T) == 8 ? 2 : 4)) void euler_row
   using C = Row3Cfg<T, L, P>;
   using PL = Row3Plan<L>;
   constexpr int NT = C::NT, E = C::E, N = C::N, NH = C::NH, TS = C::TS;
-  using FI = BlockFFT2<T, L, TS, NT, +1, C::SPLIT, 3, PL::INV_MAPF, MAP_SEQ>;
-  using FF = BlockFFT2<T, L, TS, NT, -1, C::SPLIT, 3, MAP_SEQ, MAP_PAIR>;
+  using FI = BlockFFT2<T, L, TS, NT, +1, C::SPLIT, 3, PL::INV_MAPF, MAP_SEQ, true>;
+  using FF = BlockFFT2<T, L, TS, NT, -1, C::SPLIT, 3, MAP_SEQ, MAP_PAIR, true>;
+  static_assert(FI::lds_words == C::XWORDS && FF::lds_words == C::XWORDS, "exchange size");
   constexpr int RI0 = first_radix(typename PL::Inv{});
   static_assert(last_radix(typename PL::Inv{}) == 8 && first_radix(typename PL::Fwd{}) == 8,
                 "products need the inverse's last and the forward's first pass on radix 8");
   constexpr int RL = last_radix(typename PL::Fwd{});
   __shared__ T X[C::XWORDS];
   __shared__ double red[NT / 64 + 1];
-  if (a.st->done) return;
+  const long bps = a.nlines / (2 * P);  // workgroups per sample
+  const long sample = blockIdx.x / bps;
+  DevState* st = a.st + sample;
+  if (st->done) return;
 
   const int t = threadIdx.x;
-  const long line0 = (long)blockIdx.x * (2 * P);
-  const cx<T>* __restrict__ in = (const cx<T>*)a.in;
-  cx<T>* __restrict__ out = (cx<T>*)a.out;
+  const long line0 = (blockIdx.x % bps) * (2 * P);
+  const cx<T>* __restrict__ in = (const cx<T>*)a.in + sample * a.in_ss;
+  cx<T>* __restrict__ out = (cx<T>*)a.out + sample * a.out_ss;
   const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
   const T nrm = T(a.norm);
   double lmax = 0.0;
@@ -895,10 +911,13 @@ __global__ __launch_bounds__(P * L / 8, (sizeof(T) == 8 ? 2 : 4)) void euler_row
       cx<T>* ob = out + ((long)m * a.nlines + lb) * a.ls;
       // register indices must be compile-time: the p == 0 unit (butterflies 0 and
       // H, each its own mirror) and the general unit are unrolled separately
+      // streaming stores: keep L2 for the inputs, which round 1 reads again
       auto emit = [&](int pos, const cx<T>& zk, const cx<T>& zm) {
         if (pos <= N / 2) {
-          oa[pos] = mk(T(0.5) * (zk.x + zm.x), T(0.5) * (zk.y - zm.y));
-          ob[pos] = mk(T(0.5) * (zk.y + zm.y), T(-0.5) * (zk.x - zm.x));
+          __builtin_nontemporal_store(T(0.5) * (zk.x + zm.x), &oa[pos].x);
+          __builtin_nontemporal_store(T(0.5) * (zk.y - zm.y), &oa[pos].y);
+          __builtin_nontemporal_store(T(0.5) * (zk.y + zm.y), &ob[pos].x);
+          __builtin_nontemporal_store(T(-0.5) * (zk.x - zm.x), &ob[pos].y);
         }
       };
       if (p != 0) {
@@ -923,7 +942,7 @@ __global__ __launch_bounds__(P * L / 8, (sizeof(T) == 8 ? 2 : 4)) void euler_row
   if (t == 0) {
     double mx = 0.0;
     for (int w = 0; w < (NT + 63) / 64; ++w) mx = fmax(mx, red[w]);
-    atomicMax(&a.st->umax_bins[a.slot][blockIdx.x & (kBins - 1)],
+    atomicMax(&st->umax_bins[a.slot][blockIdx.x & (kBins - 1)],
               (unsigned long long)__double_as_longlong(mx));
   }
 }

@@ -105,38 +105,38 @@ int launch_col_full(int L, ColArgs a, long batches, hipStream_t st) {
 }
 
 template <typename T, int L, int DIM>
-void launch_row_L(RowArgs a, hipStream_t st) {
+void launch_row_L(RowArgs a, long nsamples, hipStream_t st) {
   constexpr int P = (1536 / L) < 1 ? 1 : 1536 / L;
   using C = RowCfg<T, L, P, DIM>;
-  // every workgroup owns exactly P whole line pairs (checked by the caller)
-  const long blocks = a.nlines / (2 * P);
+  // every workgroup owns exactly P whole line pairs of one sample (checked by the caller)
+  const long blocks = a.nlines / (2 * P) * nsamples;
   hipLaunchKernelGGL((euler_row_kernel<T, L, P, DIM>), dim3((unsigned)blocks), dim3(C::NT), 0, st, a);
 }
 
 template <typename T, int L>
-void launch_row3_L(RowArgs a, hipStream_t st) {
+void launch_row3_L(RowArgs a, long nsamples, hipStream_t st) {
   constexpr int P = 1536 / L;
-  // every workgroup owns exactly P whole line pairs (checked by the caller)
-  const long blocks = a.nlines / (2 * P);
+  // every workgroup owns exactly P whole line pairs of one sample (checked by the caller)
+  const long blocks = a.nlines / (2 * P) * nsamples;
   hipLaunchKernelGGL((euler_row3_kernel<T, L, P>), dim3((unsigned)blocks), dim3(P * L / 8), 0, st, a);
 }
 
 template <typename T, int DIM>
-int launch_row(int L, RowArgs a, hipStream_t st) {
+int launch_row(int L, RowArgs a, long nsamples, hipStream_t st) {
   const int P = row_p(L);
   if (a.nlines % (2 * P) != 0) return fail(AZB_ERR_UNSUPPORTED, "line count not a multiple of the line-pair tile");
   if (DIM == 3) {
     switch (L) {
-      case 96: launch_row3_L<T, 96>(a, st); return AZB_OK;
-      case 192: launch_row3_L<T, 192>(a, st); return AZB_OK;
-      case 384: launch_row3_L<T, 384>(a, st); return AZB_OK;
-      case 768: launch_row3_L<T, 768>(a, st); return AZB_OK;
+      case 96: launch_row3_L<T, 96>(a, nsamples, st); return AZB_OK;
+      case 192: launch_row3_L<T, 192>(a, nsamples, st); return AZB_OK;
+      case 384: launch_row3_L<T, 384>(a, nsamples, st); return AZB_OK;
+      case 768: launch_row3_L<T, 768>(a, nsamples, st); return AZB_OK;
       default: break;
     }
   }
   switch (L) {
 #define AZB_CASE(LL) \
-  case LL: launch_row_L<T, LL, DIM>(a, st); break;
+  case LL: launch_row_L<T, LL, DIM>(a, nsamples, st); break;
     AZB_CASE(96) AZB_CASE(192) AZB_CASE(384) AZB_CASE(768) AZB_CASE(1536)
 #undef AZB_CASE
     default:
@@ -175,6 +175,7 @@ int launch_plain_row(int L, PlainRowArgs a, hipStream_t st) {
 // ---------------------------------------------------------------------------
 struct azb_solver {
   int dim = 0, N = 0, M = 0, nh = 0, nvars = 0, prec = 0;
+  int S = 1;  // samples solved together (same grid), stacked outermost
   long nhp = 0;  // last-axis stride of the intermediates: N/2+1 rounded up to 16
   azb_sv_params visc{};
   hipStream_t stream = nullptr;
@@ -189,6 +190,9 @@ struct azb_solver {
   void* tw = nullptr;
   size_t bytes = 0;
   long nmodes = 0;  // modes per component on the N-grid
+  double loop_C = 0.2;  // CFL number of the device-driven loop
+  DevState* probe_host = nullptr;  // pinned mirror of the device state (non-blocking probes)
+  hipEvent_t probe_ev = nullptr;
   // per-kernel-family hipEvent timing (off unless enabled)
   bool prof = false;
   std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> prof_ev;
@@ -234,6 +238,7 @@ template <typename T>
 int enqueue_rhs_T(azb_solver* s, const double2* u_in, const double2* u0, double2* out, int kmode,
                   int slot, double C) {
   const int N = s->N, M = s->M, nh = s->nh;
+  const long S = s->S;
   const long hp = s->nhp;  // padded last-axis stride of every intermediate
   hipStream_t st = s->stream;
   int rc;
@@ -254,42 +259,47 @@ int enqueue_rhs_T(azb_solver* s, const double2* u_in, const double2* u0, double2
     c.in_qs = nh; c.out_qs = hp;
     c.in_rs = (long)N * nh; c.out_rs = (long)N * hp;
     c.in_bs = (long)N * N * nh; c.out_bs = (long)M * N * hp;
+    c.bps = 3;
     {
       ProfScope p(s, AZB_PROF_INV_AX0);
-      if ((rc = launch_col<double, T, T, +1, COL_PAD_IN>(M, c, 3, st))) return rc;
+      if ((rc = launch_col<double, T, T, +1, COL_PAD_IN>(M, c, 3L * S, st))) return rc;
     }
     // K2: inverse along axis 1; X[c][i'][j][hp] -> Y[c][i'][j'][hp]
     c.in = s->X; c.out = s->Y; c.nq = 1;
     c.in_qs = 0; c.out_qs = 0;
     c.in_rs = hp; c.out_rs = hp;
     c.in_bs = (long)N * hp; c.out_bs = (long)M * hp;
+    c.bps = 3 * M;
     {
       ProfScope p(s, AZB_PROF_INV_AX1);
-      if ((rc = launch_col<T, T, T, +1, COL_PAD_IN>(M, c, 3L * M, st))) return rc;
+      if ((rc = launch_col<T, T, T, +1, COL_PAD_IN>(M, c, 3L * M * S, st))) return rc;
     }
     // K3: last axis c2r, products, max, r2c; Y[c][line][hp] -> Z[m][line][hp]
     r.in = s->Y; r.out = s->Z; r.nlines = (long)M * M;
+    r.in_ss = 3L * M * M * hp; r.out_ss = 6L * M * M * hp;
     {
       ProfScope p(s, AZB_PROF_ROW);
-      if ((rc = launch_row<T, 3>(M, r, st))) return rc;
+      if ((rc = launch_row<T, 3>(M, r, S, st))) return rc;
     }
     // K4: forward along axis 1, M -> N rows; Z[m][i'][j'][hp] -> X[m][i'][j][hp]
     c.in = s->Z; c.out = s->X; c.nq = 1;
     c.in_qs = 0; c.out_qs = 0;
     c.in_rs = hp; c.out_rs = hp;
     c.in_bs = (long)M * hp; c.out_bs = (long)N * hp;
+    c.bps = 6 * M;
     {
       ProfScope p(s, AZB_PROF_FWD_AX1);
-      if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 6L * M, st))) return rc;
+      if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 6L * M * S, st))) return rc;
     }
     // K5: forward along axis 0; X[m][i'][j][hp] -> Y[m][i][j][hp] (B_hat on the N-grid)
     c.in = s->X; c.out = s->Y; c.nq = N;
     c.in_qs = hp; c.out_qs = hp;
     c.in_rs = (long)N * hp; c.out_rs = (long)N * hp;
     c.in_bs = (long)M * N * hp; c.out_bs = (long)N * N * hp;
+    c.bps = 6;
     {
       ProfScope p(s, AZB_PROF_FWD_AX0);
-      if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 6, st))) return rc;
+      if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 6L * S, st))) return rc;
     }
   } else {
     // K1: inverse along axis 0; state [c][i][k] -> X[c][i'][hp]
@@ -297,28 +307,31 @@ int enqueue_rhs_T(azb_solver* s, const double2* u_in, const double2* u0, double2
     c.in_qs = 0; c.out_qs = 0;
     c.in_rs = nh; c.out_rs = hp;
     c.in_bs = (long)N * nh; c.out_bs = (long)M * hp;
+    c.bps = 2;
     {
       ProfScope p(s, AZB_PROF_INV_AX0);
-      if ((rc = launch_col<double, T, T, +1, COL_PAD_IN>(M, c, 2, st))) return rc;
+      if ((rc = launch_col<double, T, T, +1, COL_PAD_IN>(M, c, 2L * S, st))) return rc;
     }
     // K2: last axis; X[c][i'][hp] -> Z[m][i'][hp]
     r.in = s->X; r.out = s->Z; r.nlines = M;
+    r.in_ss = 2L * M * hp; r.out_ss = 3L * M * hp;
     {
       ProfScope p(s, AZB_PROF_ROW);
-      if ((rc = launch_row<T, 2>(M, r, st))) return rc;
+      if ((rc = launch_row<T, 2>(M, r, S, st))) return rc;
     }
     // K3: forward along axis 0; Z[m][i'][hp] -> Y[m][i][hp]
     c.in = s->Z; c.out = s->Y; c.nq = 1;
     c.in_qs = 0; c.out_qs = 0;
     c.in_rs = hp; c.out_rs = hp;
     c.in_bs = (long)M * hp; c.out_bs = (long)N * hp;
+    c.bps = 3;
     {
       ProfScope p(s, AZB_PROF_FWD_AX0);
-      if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 3, st))) return rc;
+      if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 3L * S, st))) return rc;
     }
   }
   const double Md = std::pow((double)M, s->dim);
-  hipLaunchKernelGGL(finalize_umax, dim3(1), dim3(64), 0, st, s->st, slot, kmode == K_RK1 ? 1 : 0,
+  hipLaunchKernelGGL(finalize_umax, dim3((unsigned)S), dim3(64), 0, st, s->st, slot, kmode == K_RK1 ? 1 : 0,
                      Md, C, (double)N);
   KArgs k{};
   k.bh = s->Y;
@@ -339,9 +352,9 @@ int enqueue_rhs_T(azb_solver* s, const double2* u_in, const double2* u0, double2
   {
     ProfScope p(s, AZB_PROF_KSPACE);
     if (s->dim == 3)
-      hipLaunchKernelGGL((kspace_kernel<T, 3>), dim3(blocks), dim3(256), 0, st, k);
+      hipLaunchKernelGGL((kspace_kernel<T, 3>), dim3(blocks, (unsigned)S), dim3(256), 0, st, k);
     else
-      hipLaunchKernelGGL((kspace_kernel<T, 2>), dim3(blocks), dim3(256), 0, st, k);
+      hipLaunchKernelGGL((kspace_kernel<T, 2>), dim3(blocks, (unsigned)S), dim3(256), 0, st, k);
   }
   HIPCHK(hipGetLastError());
   return AZB_OK;
@@ -368,23 +381,26 @@ extern "C" {
 
 const char* azb_last_error(void) { return g_err.c_str(); }
 
-int azb_create(int dim, int N, int M, int n_vars, int precision, azb_sv_params visc, void* stream,
-               azb_solver** out) {
+int azb_create_batch(int dim, int N, int M, int n_vars, int precision, azb_sv_params visc, int nsamples,
+                     void* stream, azb_solver** out) {
   if (!out) return fail(AZB_ERR_ARG, "out is NULL");
   *out = nullptr;
   if (dim != 2 && dim != 3) return fail(AZB_ERR_ARG, "dim must be 2 or 3");
   if (n_vars != dim) return fail(AZB_ERR_UNSUPPORTED, "n_vars must equal dim (no tracer)");
   if (N < 8 || N % 4 != 0) return fail(AZB_ERR_ARG, "N must be a multiple of 4");
+  if (nsamples < 1 || nsamples > 65535) return fail(AZB_ERR_ARG, "nsamples must be in [1, 65535]");
   if (M == 0) M = 3 * N / 2;
   if (M != 3 * N / 2) return fail(AZB_ERR_UNSUPPORTED, "only the default padding M = 3N/2 is built");
   if (M != 96 && M != 192 && M != 384 && M != 768 && M != 1536)
     return fail(AZB_ERR_UNSUPPORTED, "padded size " + std::to_string(M) + " not built");
+  if (dim == 3 && M == 1536) return fail(AZB_ERR_UNSUPPORTED, "3D N = 1024 is not built");
   if (precision != AZB_FP64 && precision != AZB_FP32_MIXED)
     return fail(AZB_ERR_ARG, "unknown precision mode");
   azb_solver* s = new azb_solver();
   s->dim = dim;
   s->N = N;
   s->M = M;
+  s->S = nsamples;
   s->nh = N / 2 + 1;
   s->nvars = n_vars;
   s->prec = precision;
@@ -394,19 +410,20 @@ int azb_create(int dim, int N, int M, int n_vars, int precision, azb_sv_params v
   const long nh = s->nh;
   s->nhp = (nh + 15) / 16 * 16;
   const long hp = s->nhp;
+  const size_t S = nsamples;
   size_t xs, ys, zs;
   if (dim == 3) {
     s->nmodes = (long)N * N * nh;
-    xs = cs * 6 * (size_t)M * N * hp;                                   // T1 (3) / S2 (6)
-    ys = cs * std::max<size_t>(3 * (size_t)M * M * hp, 6 * (size_t)N * N * hp);  // T2 / B_hat
-    zs = cs * 6 * (size_t)M * M * hp;                                   // S1
+    xs = S * cs * 6 * (size_t)M * N * hp;                                   // T1 (3) / S2 (6)
+    ys = S * cs * std::max<size_t>(3 * (size_t)M * M * hp, 6 * (size_t)N * N * hp);  // T2 / B_hat
+    zs = S * cs * 6 * (size_t)M * M * hp;                                   // S1
   } else {
     s->nmodes = (long)N * nh;
-    xs = cs * 2 * (size_t)M * hp;
-    ys = cs * 3 * (size_t)N * hp;
-    zs = cs * 3 * (size_t)M * hp;
+    xs = S * cs * 2 * (size_t)M * hp;
+    ys = S * cs * 3 * (size_t)N * hp;
+    zs = S * cs * 3 * (size_t)M * hp;
   }
-  const size_t us = sizeof(double2) * (size_t)dim * s->nmodes;
+  const size_t us = S * sizeof(double2) * (size_t)dim * s->nmodes;
   auto cleanup = [&](int code, const std::string& m) {
     azb_destroy(s);
     return fail(code, m);
@@ -414,20 +431,41 @@ int azb_create(int dim, int N, int M, int n_vars, int precision, azb_sv_params v
   if (hipMalloc(&s->X, xs) != hipSuccess || hipMalloc(&s->Y, ys) != hipSuccess ||
       hipMalloc(&s->Z, zs) != hipSuccess || hipMalloc((void**)&s->u1, us) != hipSuccess ||
       hipMalloc((void**)&s->u2, us) != hipSuccess ||
-      hipMalloc((void**)&s->st, sizeof(DevState)) != hipSuccess)
+      hipMalloc((void**)&s->st, S * sizeof(DevState)) != hipSuccess)
     return cleanup(AZB_ERR_ALLOC, "device allocation of the solver workspace failed");
-  s->bytes = xs + ys + zs + 2 * us + sizeof(DevState);
+  s->bytes = xs + ys + zs + 2 * us + S * sizeof(DevState);
   int rc = precision == AZB_FP64 ? make_twiddles<double>(M, &s->tw) : make_twiddles<float>(M, &s->tw);
   if (rc) {
     std::string m = g_err;
     return cleanup(rc, m);
   }
   s->stream = (hipStream_t)stream;  // NULL: the default (null) stream
-  DevState h{};
-  if (hipMemcpy(s->st, &h, sizeof(DevState), hipMemcpyHostToDevice) != hipSuccess)
+  std::vector<DevState> h(S);
+  if (hipMemcpy(s->st, h.data(), S * sizeof(DevState), hipMemcpyHostToDevice) != hipSuccess)
     return cleanup(AZB_ERR_HIP, "hipMemcpy of the device state failed");
   *out = s;
   return AZB_OK;
+}
+
+int azb_create(int dim, int N, int M, int n_vars, int precision, azb_sv_params visc, void* stream,
+               azb_solver** out) {
+  return azb_create_batch(dim, N, M, n_vars, precision, visc, 1, stream, out);
+}
+
+int azb_num_samples(const azb_solver* s) { return s ? s->S : 0; }
+
+size_t azb_bytes_estimate(int dim, int N, int precision, int nsamples) {
+  const size_t cs = precision == AZB_FP64 ? 16 : 8;
+  const size_t M = 3 * (size_t)N / 2, nh = N / 2 + 1, hp = (nh + 15) / 16 * 16, S = nsamples;
+  size_t ws, us;
+  if (dim == 3) {
+    ws = cs * (6 * M * N * hp + std::max(3 * M * M * hp, 6 * (size_t)N * N * hp) + 6 * M * M * hp);
+    us = sizeof(double2) * 3 * (size_t)N * N * nh;
+  } else {
+    ws = cs * (2 * M * hp + 3 * (size_t)N * hp + 3 * M * hp);
+    us = sizeof(double2) * 2 * (size_t)N * nh;
+  }
+  return S * (ws + 2 * us + sizeof(DevState));
 }
 
 int azb_destroy(azb_solver* s) {
@@ -440,6 +478,8 @@ int azb_destroy(azb_solver* s) {
   (void)hipFree(s->u2);
   (void)hipFree(s->st);
   (void)hipFree(s->tw);
+  if (s->probe_host) (void)hipHostFree(s->probe_host);
+  if (s->probe_ev) (void)hipEventDestroy(s->probe_ev);
   for (auto& e : s->prof_ev) {
     (void)hipEventDestroy(e.second.first);
     (void)hipEventDestroy(e.second.second);
@@ -455,7 +495,7 @@ void* azb_stream(const azb_solver* s) { return s ? (void*)s->stream : nullptr; }
 
 int azb_dudt(azb_solver* s, const void* u_hat, void* dudt_hat) {
   if (!s || !u_hat || !dudt_hat) return fail(AZB_ERR_ARG, "NULL argument");
-  hipLaunchKernelGGL(step_prologue_fixed, dim3(1), dim3(256), 0, s->stream, s->st, 0.0);
+  hipLaunchKernelGGL(step_prologue_fixed, dim3(s->S), dim3(256), 0, s->stream, s->st, 0.0);
   return enqueue_rhs(s, (const double2*)u_hat, nullptr, (double2*)dudt_hat, K_DUDT, 3, 0.0);
 }
 
@@ -477,7 +517,7 @@ int azb_dt(azb_solver* s, double C, double* dt_host) {
 
 int azb_rk3_step(azb_solver* s, void* u_hat, double max_dt, double C, double* dt_host) {
   if (!s || !u_hat) return fail(AZB_ERR_ARG, "NULL argument");
-  hipLaunchKernelGGL(step_prologue_fixed, dim3(1), dim3(256), 0, s->stream, s->st, max_dt);
+  hipLaunchKernelGGL(step_prologue_fixed, dim3(s->S), dim3(256), 0, s->stream, s->st, max_dt);
   int rc = enqueue_rk3_stages(s, (double2*)u_hat, C);
   if (rc) return rc;
   if (dt_host) {
@@ -487,41 +527,175 @@ int azb_rk3_step(azb_solver* s, void* u_hat, double max_dt, double C, double* dt
   return AZB_OK;
 }
 
+int azb_loop_begin(azb_solver* s, double t_end, double C, const double* times0_host) {
+  if (!s) return fail(AZB_ERR_ARG, "NULL solver");
+  const double pi = 3.14159265358979323846;
+  std::vector<DevState> h(s->S);
+  for (int i = 0; i < s->S; ++i) {
+    h[i].time = times0_host ? times0_host[i] : 0.0;
+    h[i].t_end = t_end;
+    h[i].max_dt_visc = C * 2. / (s->visc.eps * ((pi * s->N) * (pi * s->N)));
+    h[i].done = 0;
+    h[i].steps = 0;
+  }
+  s->loop_C = C;
+  HIPCHK(hipMemcpyAsync(s->st, h.data(), s->S * sizeof(DevState), hipMemcpyHostToDevice, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  return AZB_OK;
+}
+
+int azb_enqueue_steps(azb_solver* s, void* u_hat, int nsteps) {
+  if (!s || !u_hat) return fail(AZB_ERR_ARG, "NULL argument");
+  for (int i = 0; i < nsteps; ++i) {
+    hipLaunchKernelGGL(step_prologue_loop, dim3(s->S), dim3(256), 0, s->stream, s->st);
+    int rc = enqueue_rk3_stages(s, (double2*)u_hat, s->loop_C);
+    if (rc) return rc;
+    hipLaunchKernelGGL(step_epilogue_loop, dim3(s->S), dim3(64), 0, s->stream, s->st);
+  }
+  HIPCHK(hipGetLastError());
+  return AZB_OK;
+}
+
+int azb_loop_read(azb_solver* s, double* times_host, int* steps_host) {
+  if (!s) return fail(AZB_ERR_ARG, "NULL solver");
+  std::vector<DevState> h(s->S);
+  HIPCHK(hipMemcpyAsync(h.data(), s->st, s->S * sizeof(DevState), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipStreamSynchronize(s->stream));
+  for (int i = 0; i < s->S; ++i) {
+    if (times_host) times_host[i] = h[i].time;
+    if (steps_host) steps_host[i] = h[i].steps;
+  }
+  return AZB_OK;
+}
+
+int azb_loop_probe(azb_solver* s) {
+  if (!s) return fail(AZB_ERR_ARG, "NULL solver");
+  if (!s->probe_host) {
+    HIPCHK(hipHostMalloc((void**)&s->probe_host, s->S * sizeof(DevState), hipHostMallocDefault));
+    HIPCHK(hipEventCreateWithFlags(&s->probe_ev, hipEventDisableTiming));
+  }
+  HIPCHK(hipMemcpyAsync(s->probe_host, s->st, s->S * sizeof(DevState), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(hipEventRecord(s->probe_ev, s->stream));
+  return AZB_OK;
+}
+
+int azb_loop_probe_result(azb_solver* s, int* ready, double* times_host, int* steps_host) {
+  if (!s || !ready) return fail(AZB_ERR_ARG, "NULL argument");
+  if (!s->probe_ev) return fail(AZB_ERR_ARG, "no probe queued");
+  const hipError_t e = hipEventQuery(s->probe_ev);
+  if (e == hipErrorNotReady) {
+    *ready = 0;
+    return AZB_OK;
+  }
+  if (e != hipSuccess) return fail(AZB_ERR_HIP, std::string("hipEventQuery: ") + hipGetErrorString(e));
+  *ready = 1;
+  for (int i = 0; i < s->S; ++i) {
+    if (times_host) times_host[i] = s->probe_host[i].time;
+    if (steps_host) steps_host[i] = s->probe_host[i].steps;
+  }
+  return AZB_OK;
+}
+
+int azb_memcpy_h2d_async(void* dst, const void* src_host, size_t bytes, void* stream) {
+  HIPCHK(hipMemcpyAsync(dst, src_host, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+  return AZB_OK;
+}
+
+int azb_memcpy_d2h_async(void* dst_host, const void* src, size_t bytes, void* stream) {
+  HIPCHK(hipMemcpyAsync(dst_host, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+  return AZB_OK;
+}
+
 int azb_simulate_until(azb_solver* s, void* u_hat, double t_end, double C, double* time_inout,
                        int max_steps, int chunk, int* steps_out) {
   if (!s || !u_hat || !time_inout) return fail(AZB_ERR_ARG, "NULL argument");
   if (chunk <= 0) chunk = 8;
-  const double pi = 3.14159265358979323846;
-  DevState h{};
-  h.time = *time_inout;
-  h.t_end = t_end;
-  h.max_dt_visc = C * 2. / (s->visc.eps * ((pi * s->N) * (pi * s->N)));
-  h.done = 0;
-  h.steps = 0;
-  HIPCHK(hipMemcpyAsync(s->st, &h, sizeof(DevState), hipMemcpyHostToDevice, s->stream));
+  std::vector<double> t0(s->S, *time_inout), times(s->S);
+  std::vector<int> steps(s->S);
+  int rc = azb_loop_begin(s, t_end, C, t0.data());
+  if (rc) return rc;
   int taken = 0;
-  struct {
-    double time;
-    int done, steps;
-  } probe{};
   while (true) {
     int n = chunk;
     if (max_steps > 0 && taken + n > max_steps) n = max_steps - taken;
-    for (int i = 0; i < n; ++i) {
-      hipLaunchKernelGGL(step_prologue_loop, dim3(1), dim3(256), 0, s->stream, s->st);
-      int rc = enqueue_rk3_stages(s, (double2*)u_hat, C);
-      if (rc) return rc;
-      hipLaunchKernelGGL(step_epilogue_loop, dim3(1), dim3(64), 0, s->stream, s->st);
-    }
+    if ((rc = azb_enqueue_steps(s, u_hat, n))) return rc;
     taken += n;
-    HIPCHK(hipMemcpyAsync(&probe.time, &s->st->time, sizeof(double), hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(hipMemcpyAsync(&probe.steps, &s->st->steps, sizeof(int), hipMemcpyDeviceToHost, s->stream));
-    HIPCHK(hipStreamSynchronize(s->stream));
-    if (!(probe.time < t_end)) break;
+    if ((rc = azb_loop_read(s, times.data(), steps.data()))) return rc;
+    bool all_done = true;
+    for (double tt : times) all_done = all_done && !(tt < t_end);
+    if (all_done) break;
     if (max_steps > 0 && taken >= max_steps) break;
   }
-  if (steps_out) *steps_out = probe.steps;
-  *time_inout = (probe.time < t_end) ? probe.time : t_end;
+  int mx = 0;
+  double tmin = t_end;
+  for (int i = 0; i < s->S; ++i) {
+    mx = std::max(mx, steps[i]);
+    tmin = std::min(tmin, times[i]);
+  }
+  if (steps_out) *steps_out = mx;
+  *time_inout = tmin;
+  return AZB_OK;
+}
+
+// ---- hipGraph capture of enqueued work -------------------------------------
+struct azb_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
+int azb_capture_begin(void* stream) {
+  if (!stream) return fail(AZB_ERR_ARG, "capture needs a non-null stream");
+  HIPCHK(hipStreamBeginCapture((hipStream_t)stream, hipStreamCaptureModeThreadLocal));
+  return AZB_OK;
+}
+
+int azb_capture_end(void* stream, azb_graph** out) {
+  if (!stream || !out) return fail(AZB_ERR_ARG, "NULL argument");
+  azb_graph* g = new azb_graph();
+  hipError_t e = hipStreamEndCapture((hipStream_t)stream, &g->graph);
+  if (e != hipSuccess) {
+    delete g;
+    return fail(AZB_ERR_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+  }
+  e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    (void)hipGraphDestroy(g->graph);
+    delete g;
+    return fail(AZB_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  }
+  *out = g;
+  return AZB_OK;
+}
+
+int azb_graph_launch(azb_graph* g, void* stream) {
+  if (!g) return fail(AZB_ERR_ARG, "NULL graph");
+  HIPCHK(hipGraphLaunch(g->exec, (hipStream_t)stream));
+  return AZB_OK;
+}
+
+int azb_graph_destroy(azb_graph* g) {
+  if (!g) return AZB_OK;
+  (void)hipGraphExecDestroy(g->exec);
+  (void)hipGraphDestroy(g->graph);
+  delete g;
+  return AZB_OK;
+}
+
+int azb_stream_create(void** stream) {
+  if (!stream) return fail(AZB_ERR_ARG, "NULL argument");
+  hipStream_t st = nullptr;
+  HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  *stream = (void*)st;
+  return AZB_OK;
+}
+
+int azb_stream_destroy(void* stream) {
+  HIPCHK(hipStreamDestroy((hipStream_t)stream));
+  return AZB_OK;
+}
+
+int azb_stream_synchronize(void* stream) {
+  HIPCHK(hipStreamSynchronize((hipStream_t)stream));
   return AZB_OK;
 }
 
@@ -634,6 +808,7 @@ int fft_T(int dim, int n, int howmany, bool forward, void* in, void* out, hipStr
       c.in_qs = c.out_qs = 0;
       c.in_rs = c.out_rs = inner;
       c.in_bs = c.out_bs = inner * n;
+      c.bps = (int)batches;  // a single device state: every batch maps to it
       int r = dir < 0 ? launch_col_full<T, -1>(n, c, batches, st) : launch_col_full<T, +1>(n, c, batches, st);
       if (r) return r;
     }

@@ -108,16 +108,108 @@ def cpu_baseline(dim: int, budget_s: float):
     }
 
 
+def run_ensemble(args):
+    """Batched 2D ensemble: discontinuous vortex patches, 96 @ 128^2, 96 @ 256^2,
+    64 @ 512^2, to t = 1, batched (one graph replay per chunk of steps) vs one
+    sample at a time through the single-solve path (timed on a subset per size)."""
+    import torch
+
+    from azeban_amd import init as I
+    from azeban_amd import initialize
+    from azeban_amd.ensemble import Ensemble, Group, solve_one
+
+    counts = {128: 96, 256: 96, 512: 64}
+    t_end = args.t_end if args.t_end > 0 else 1.0
+    out = {"metric": METRIC, "unit": "samples/s", "higher_is_better": True, "n_gpus": 1,
+           "data": "synthetic", "config": {"workload": "batched 2D ensemble, discontinuous vortex patch "
+                                           "(radius 0.2, 16-mode 0.01 perturbation), 96@128^2 + 96@256^2 + "
+                                           f"64@512^2, t={t_end}, C=0.2", "samples": sum(counts.values())}}
+    for precision in (["fp64", "fp32_mixed"] if not args.no_mixed else ["fp64"]):
+        log(f"ensemble {precision}: initial data")
+        groups = []
+        seed = 0
+        for N, S in counts.items():
+            us = []
+            for _ in range(S):
+                us.append(initialize(I.vortex_patch(N, seed=seed)))
+                seed += 1
+            groups.append(Group(2, N, torch.stack(us).contiguous(), precision))
+        init = [g.u.clone() for g in groups]
+        ens = Ensemble(groups)
+        log(f"ensemble {precision}: batched run")
+        res = ens.run(t_end, steps_per_launch=args.ens_chunk)
+        torch.cuda.synchronize()
+        samples = sum(counts.values())
+        rec = {"samples_per_s": round(samples / res["wall_s"], 3), "wall_s": round(res["wall_s"], 3),
+               "graph_launches": res["launches"],
+               "steps_per_size": {str(g.N): max(st) for g, st in zip(groups, res["steps"])}}
+        if precision == "fp64":
+            log("ensemble fp64: one-at-a-time subset")
+            per = args.ens_subset
+            t_single = 0.0
+            identical = True
+            for gi, g in enumerate(groups):
+                t0 = time.perf_counter()
+                for si in range(per):
+                    ref, n = solve_one(2, g.N, init[gi][si], t_end)
+                    torch.cuda.synchronize()
+                    identical = identical and bool(torch.equal(ref, g.u[si])) and n == res["steps"][gi][si]
+                dt = (time.perf_counter() - t0) / per
+                t_single += dt * counts[g.N]
+            rec["one_at_a_time_samples_per_s"] = round(samples / t_single, 3)
+            rec["one_at_a_time_sample"] = f"{per} samples per size timed, scaled by the counts"
+            rec["speedup_vs_one_at_a_time"] = round(rec["samples_per_s"] / rec["one_at_a_time_samples_per_s"], 3)
+            rec["identical_per_sample_results"] = identical
+            out["value"] = rec["samples_per_s"]
+            out["dtype"] = "f64"
+        out[precision] = rec
+        ens.close()
+    print(json.dumps(out), flush=True)
+
+
+def run_serving(args):
+    """Poisson arrivals (mean rate 50/s) of 2D 128^2-1024^2 and 3D 64^3-256^3 solves of
+    fractional-Brownian data (H = 0.75) to t = 0.5, continuous batching by HBM footprint."""
+    import torch
+
+    from azeban_amd.serving import Server, poisson_requests
+
+    t_end = args.t_end if args.t_end > 0 else 0.5
+    log(f"serving: generating {args.requests} requests")
+    reqs = poisson_requests(args.requests, 50.0, seed=1, t_end=t_end)
+    srv = Server(hbm_budget=int(args.hbm_gib * 2**30), max_active=args.max_active, chunk_steps=args.ens_chunk)
+    log("serving: running")
+    st = srv.run(reqs)
+    torch.cuda.synchronize()
+    out = {"metric": METRIC, "unit": "requests/s", "higher_is_better": True, "n_gpus": 1, "dtype": "f64",
+           "data": "synthetic", "value": round(st["throughput_per_s"], 3),
+           "latency_p50_s": round(st["latency_p50_s"], 4), "latency_p99_s": round(st["latency_p99_s"], 4),
+           "wall_s": round(st["wall_s"], 3), "peak_active": st["peak_active"],
+           "Mpts_RK_steps_per_s": round(st["grid_point_steps"] / st["wall_s"] / 1e6, 2),
+           "config": {"workload": "serving: Poisson arrivals mean 50/s, 2D 128^2-1024^2 and 3D 64^3-256^3, "
+                                  f"fBm H=0.75, t={t_end}, FP64", "requests": args.requests,
+                      "hbm_budget_GiB": args.hbm_gib, "max_active": args.max_active}}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["tg3d", "dsl2d"], default="tg3d")
+    ap.add_argument("--workload", choices=["tg3d", "dsl2d", "ensemble", "serving"], default="tg3d")
+    ap.add_argument("--t-end", type=float, default=0.0)
+    ap.add_argument("--ens-chunk", type=int, default=16)
+    ap.add_argument("--ens-subset", type=int, default=8)
+    ap.add_argument("--requests", type=int, default=40)
+    ap.add_argument("--hbm-gib", type=float, default=200.0)
+    ap.add_argument("--max-active", type=int, default=32)
     ap.add_argument("--N", type=int, default=0)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--calib-copy-gib", type=float, default=0.0,
+                    help="run one device-to-device copy of this size first (counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -129,6 +221,21 @@ def main():
 
     import azeban_amd as A
     from azeban_amd import traffic
+
+    if args.calib_copy_gib > 0:
+        n = int(args.calib_copy_gib * 2**30) // 8
+        xa = torch.ones(n, dtype=torch.float64, device="cuda")
+        ya = torch.empty_like(xa)
+        torch.cuda.synchronize()
+        ya.copy_(xa)
+        torch.cuda.synchronize()
+        log(f"calibration copy of {n * 8} bytes done")
+        del xa, ya
+
+    if args.workload == "ensemble":
+        return run_ensemble(args)
+    if args.workload == "serving":
+        return run_serving(args)
 
     dim = 3 if args.workload == "tg3d" else 2
     N = args.N or (512 if dim == 3 else 1024)

@@ -60,7 +60,18 @@ const char* azb_last_error(void);
  * (null) stream. */
 int azb_create(int dim, int N, int M, int n_vars, int precision, azb_sv_params visc,
                void* stream, azb_solver** out);
+/* The same for `nsamples` independent solves of one grid size, stacked as
+ * (nsamples, n_vars, N, ..., N/2+1): every kernel launch covers all samples, and
+ * each sample keeps its own max|u|, dt, time and completion flag on the device.
+ * This batches azeban's sequential sample loop (src/azeban/run_from_config.cpp:81-102)
+ * into one launch per kernel.  Per-sample results are identical to solving each
+ * sample alone. */
+int azb_create_batch(int dim, int N, int M, int n_vars, int precision, azb_sv_params visc,
+                     int nsamples, void* stream, azb_solver** out);
 int azb_destroy(azb_solver* s);
+int azb_num_samples(const azb_solver* s);
+/* Device bytes azb_create_batch would allocate (admission control). */
+size_t azb_bytes_estimate(int dim, int N, int precision, int nsamples);
 
 /* Bytes of device memory held by the solver (workspace + stage buffers + twiddles). */
 size_t azb_workspace_bytes(const azb_solver* s);
@@ -91,6 +102,31 @@ int azb_rk3_step(azb_solver* s, void* u_hat, double max_dt, double C, double* dt
  * the time reached).  Synchronising. */
 int azb_simulate_until(azb_solver* s, void* u_hat, double t_end, double C, double* time_inout,
                        int max_steps, int chunk, int* steps_out);
+
+/* ---- device-driven time loop, piecewise (for graph capture and overlap) ----
+ * azb_loop_begin sets every sample's time (times0_host[i], or 0 if NULL) and target
+ * t_end (synchronising); azb_enqueue_steps queues n steps without synchronising (a
+ * step past t_end does nothing on the device, so extra steps are harmless);
+ * azb_loop_read copies each sample's time and step count back (synchronising). */
+int azb_loop_begin(azb_solver* s, double t_end, double C, const double* times0_host);
+int azb_enqueue_steps(azb_solver* s, void* u_hat, int nsteps);
+int azb_loop_read(azb_solver* s, double* times_host, int* steps_host);
+/* Non-blocking variant: azb_loop_probe queues a copy of every sample's time and
+ * step count into pinned host memory; azb_loop_probe_result sets *ready = 0 until
+ * that copy has landed, then *ready = 1 and fills the arrays. */
+int azb_loop_probe(azb_solver* s);
+int azb_loop_probe_result(azb_solver* s, int* ready, double* times_host, int* steps_host);
+
+/* ---- hipGraph capture of work queued on a stream (solvers created on it) ---- */
+typedef struct azb_graph azb_graph;
+int azb_capture_begin(void* stream);
+int azb_capture_end(void* stream, azb_graph** out);
+int azb_graph_launch(azb_graph* g, void* stream);
+int azb_graph_destroy(azb_graph* g);
+/* non-blocking HIP stream at default priority, for capture and overlap */
+int azb_stream_create(void** stream);
+int azb_stream_destroy(void* stream);
+int azb_stream_synchronize(void* stream);
 
 /* ---- per-kernel-family device timing (hipEvent pairs on the solver's stream),
  * the analogue of azeban's ProfileDevice records (include/azeban/profiler.hpp:35-135). */
@@ -146,6 +182,8 @@ int azb_malloc(void** ptr, size_t bytes);
 int azb_free(void* ptr);
 int azb_memcpy_h2d(void* dst, const void* src_host, size_t bytes);
 int azb_memcpy_d2h(void* dst_host, const void* src, size_t bytes);
+int azb_memcpy_h2d_async(void* dst, const void* src_host, size_t bytes, void* stream);
+int azb_memcpy_d2h_async(void* dst_host, const void* src, size_t bytes, void* stream);
 int azb_synchronize(azb_solver* s);
 int azb_device_synchronize(void);
 
