@@ -873,11 +873,24 @@ __global__ __launch_bounds__(P * L / 8, (sizeof(T) == 8 ? 2 : 4)) void euler_row
     return mk(X0.x, Y0.x);
   };
 
+  // FP32 keeps the inverse output in registers for the second product round
+  // (48 more VGPRs); FP64 would exceed the register file and recomputes it
+  constexpr bool KEEP_U = sizeof(T) == 4;
   cx<T> v[E];
+  cx<T> ukeep[KEEP_U ? E : 1];
 #pragma unroll 1
   for (int round = 0; round < 2; ++round) {
-    FI::template load_dft<RI0>(v, t, ld_inv);
-    FI::run_rest(v, t, X, tw, typename PL::Inv{});
+    if (!KEEP_U || round == 0) {
+      FI::template load_dft<RI0>(v, t, ld_inv);
+      FI::run_rest(v, t, X, tw, typename PL::Inv{});
+      if (KEEP_U) {
+#pragma unroll
+        for (int e = 0; e < E; ++e) ukeep[KEEP_U ? e : 0] = v[e];
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < E; ++e) v[e] = ukeep[KEEP_U ? e : 0];
+    }
     // v[g*8 + r]: packed sequence type g of this thread's pair, same 8 positions
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
