@@ -75,6 +75,9 @@ SIGNATURES = {
     "azb_profile_enable": (_i, [_vp, _i]),
     "azb_profile_read": (_i, [_vp, _dp, C.POINTER(C.c_long), _i]),
     "azb_profile_reset": (_i, [_vp]),
+    "azb_autotune": (_i, [_vp, _vp, _i]),
+    "azb_set_tuning": (_i, [_vp, _ip, _i]),
+    "azb_get_tuning": (_i, [_vp, _ip, _ip]),
     "azb_synchronize": (_i, [_vp]),
     "azb_device_synchronize": (_i, []),
 }

@@ -98,7 +98,7 @@ constexpr int col_ts() {
 }
 
 template <typename Tin, typename Tout, typename T, int L, int TS, int DIR, int MODE>
-__global__ __launch_bounds__(TS * L / LenPlan<L>::E) void col_fft_kernel(ColArgs a) {
+__global__ __launch_bounds__(TS * L / LenPlan<L>::E, (sizeof(T) == 4 ? 4 : (MODE == COL_PAD_IN && (L == 768 || L == 384) ? 3 : 2))) void col_fft_kernel(ColArgs a) {
   using LP = LenPlan<L>;
   constexpr int E = LP::E;
   constexpr int NT = TS * L / E;
@@ -123,7 +123,8 @@ __global__ __launch_bounds__(TS * L / LenPlan<L>::E) void col_fft_kernel(ColArgs
   const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
 
   cx<T> v[E];
-  F::template load_dft<R0>(v, t, [&](int s, int pos) -> cx<T> {
+  // all first-pass loads are issued before any use (maximum bytes in flight)
+  F::template load<R0>(v, t, [&](int s, int pos) -> cx<T> {
     long row = pos;
     if (MODE == COL_PAD_IN) {
       if (pos <= half)
@@ -140,7 +141,7 @@ __global__ __launch_bounds__(TS * L / LenPlan<L>::E) void col_fft_kernel(ColArgs
     }
     return r;
   });
-  F::run_rest(v, t, lds, tw, typename LP::P{});
+  F::run(v, t, lds, tw, typename LP::P{});
   F::template store<RL>(v, t, [&](int s, int pos, cx<T> val) {
     long row = pos;
     if (MODE == COL_TRUNC_OUT) {

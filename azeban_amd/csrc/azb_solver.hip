@@ -66,21 +66,35 @@ int make_twiddles(int L, void** out) {
 // ---------------------------------------------------------------------------
 // launch dispatch over the compile-time transform length
 // ---------------------------------------------------------------------------
-template <typename Tin, typename Tout, typename T, int L, int DIR, int MODE>
-void launch_col_L(ColArgs a, long batches, hipStream_t st) {
+template <typename Tin, typename Tout, typename T, int L, int TS, int DIR, int MODE>
+void launch_col_TS(ColArgs a, long batches, hipStream_t st) {
   constexpr int E = LenPlan<L>::E;
-  constexpr int TS = azb::col_ts<T, L>();
   a.ktiles = (a.nk + TS - 1) / TS;
   const long blocks = batches * a.nq * a.ktiles;
   hipLaunchKernelGGL((col_fft_kernel<Tin, Tout, T, L, TS, DIR, MODE>), dim3((unsigned)blocks),
                      dim3(TS * L / E), 0, st, a);
 }
 
+// tsdiv in {1, 2, 4} selects tiles of TS0, TS0/2 or TS0/4 columns, TS0 being one
+// 128-byte line: narrower tiles give small grids more workgroups
+template <typename Tin, typename Tout, typename T, int L, int DIR, int MODE>
+void launch_col_L(ColArgs a, long batches, hipStream_t st, int tsdiv = 1) {
+  constexpr int TS0 = azb::col_ts<T, L>();
+  constexpr int TS1 = TS0 / 2 >= 1 ? TS0 / 2 : 1;
+  constexpr int TS2 = TS0 / 4 >= 1 ? TS0 / 4 : 1;
+  if (tsdiv >= 4)
+    launch_col_TS<Tin, Tout, T, L, TS2, DIR, MODE>(a, batches, st);
+  else if (tsdiv >= 2)
+    launch_col_TS<Tin, Tout, T, L, TS1, DIR, MODE>(a, batches, st);
+  else
+    launch_col_TS<Tin, Tout, T, L, TS0, DIR, MODE>(a, batches, st);
+}
+
 template <typename Tin, typename Tout, typename T, int DIR, int MODE>
-int launch_col(int L, ColArgs a, long batches, hipStream_t st) {
+int launch_col(int L, ColArgs a, long batches, hipStream_t st, int tsdiv = 1) {
   switch (L) {
 #define AZB_CASE(LL) \
-  case LL: launch_col_L<Tin, Tout, T, LL, DIR, MODE>(a, batches, st); break;
+  case LL: launch_col_L<Tin, Tout, T, LL, DIR, MODE>(a, batches, st, tsdiv); break;
     AZB_CASE(96) AZB_CASE(192) AZB_CASE(384) AZB_CASE(768) AZB_CASE(1536)
 #undef AZB_CASE
     default:
@@ -93,7 +107,7 @@ template <typename T, int DIR>
 int launch_col_full(int L, ColArgs a, long batches, hipStream_t st) {
   switch (L) {
 #define AZB_CASE(LL) \
-  case LL: launch_col_L<T, T, T, LL, DIR, COL_FULL>(a, batches, st); break;
+  case LL: launch_col_TS<T, T, T, LL, azb::col_ts<T, LL>(), DIR, COL_FULL>(a, batches, st); break;
     AZB_CASE(16) AZB_CASE(32) AZB_CASE(64) AZB_CASE(128) AZB_CASE(256) AZB_CASE(512)
     AZB_CASE(1024) AZB_CASE(48) AZB_CASE(96) AZB_CASE(192) AZB_CASE(384) AZB_CASE(768)
     AZB_CASE(1536)
@@ -104,41 +118,55 @@ int launch_col_full(int L, ColArgs a, long batches, hipStream_t st) {
   return AZB_OK;
 }
 
-template <typename T, int L, int DIM>
-void launch_row_L(RowArgs a, long nsamples, hipStream_t st) {
-  constexpr int P = (1536 / L) < 1 ? 1 : 1536 / L;
+template <typename T, int L, int P, int DIM>
+void launch_row_P(RowArgs a, long nsamples, hipStream_t st) {
   using C = RowCfg<T, L, P, DIM>;
   // every workgroup owns exactly P whole line pairs of one sample (checked by the caller)
   const long blocks = a.nlines / (2 * P) * nsamples;
   hipLaunchKernelGGL((euler_row_kernel<T, L, P, DIM>), dim3((unsigned)blocks), dim3(C::NT), 0, st, a);
 }
 
-template <typename T, int L>
-void launch_row3_L(RowArgs a, long nsamples, hipStream_t st) {
-  constexpr int P = 1536 / L;
-  // every workgroup owns exactly P whole line pairs of one sample (checked by the caller)
+template <typename T, int L, int P>
+void launch_row3_P(RowArgs a, long nsamples, hipStream_t st) {
   const long blocks = a.nlines / (2 * P) * nsamples;
   hipLaunchKernelGGL((euler_row3_kernel<T, L, P>), dim3((unsigned)blocks), dim3(P * L / 8), 0, st, a);
 }
 
-template <typename T, int DIM>
-int launch_row(int L, RowArgs a, long nsamples, hipStream_t st) {
-  const int P = row_p(L);
-  if (a.nlines % (2 * P) != 0) return fail(AZB_ERR_UNSUPPORTED, "line count not a multiple of the line-pair tile");
-  if (DIM == 3) {
-    switch (L) {
-      case 96: launch_row3_L<T, 96>(a, nsamples, st); return AZB_OK;
-      case 192: launch_row3_L<T, 192>(a, nsamples, st); return AZB_OK;
-      case 384: launch_row3_L<T, 384>(a, nsamples, st); return AZB_OK;
-      case 768: launch_row3_L<T, 768>(a, nsamples, st); return AZB_OK;
-      default: break;
-    }
+// pdiv in {1, 2}: P0 = 1536/L line pairs per workgroup, or half of that
+template <typename T, int L, int DIM>
+void launch_row_L(RowArgs a, long nsamples, hipStream_t st, int pdiv) {
+  constexpr int P0 = (1536 / L) < 1 ? 1 : 1536 / L;
+  constexpr int P1 = P0 / 2 >= 1 ? P0 / 2 : 1;
+  if constexpr (DIM == 3) {
+    if (pdiv >= 2)
+      launch_row3_P<T, L, P1>(a, nsamples, st);
+    else
+      launch_row3_P<T, L, P0>(a, nsamples, st);
+  } else {
+    if (pdiv >= 2)
+      launch_row_P<T, L, P1, DIM>(a, nsamples, st);
+    else
+      launch_row_P<T, L, P0, DIM>(a, nsamples, st);
   }
+}
+
+template <typename T, int DIM>
+int launch_row(int L, RowArgs a, long nsamples, hipStream_t st, int pdiv = 1) {
+  int P = row_p(L);
+  if (pdiv >= 2 && P >= 2) P /= 2;
+  if (a.nlines % (2 * P) != 0) return fail(AZB_ERR_UNSUPPORTED, "line count not a multiple of the line-pair tile");
+  if (DIM == 3 && L == 1536) return fail(AZB_ERR_UNSUPPORTED, "3D row length 1536 not built");
   switch (L) {
 #define AZB_CASE(LL) \
-  case LL: launch_row_L<T, LL, DIM>(a, nsamples, st); break;
-    AZB_CASE(96) AZB_CASE(192) AZB_CASE(384) AZB_CASE(768) AZB_CASE(1536)
+  case LL: launch_row_L<T, LL, DIM>(a, nsamples, st, pdiv); break;
+    AZB_CASE(96) AZB_CASE(192) AZB_CASE(384) AZB_CASE(768)
 #undef AZB_CASE
+    case 1536:
+      if (DIM == 2) {
+        launch_row_L<T, 1536, 2>(a, nsamples, st, pdiv);
+        break;
+      }
+      return fail(AZB_ERR_UNSUPPORTED, "3D row length 1536 not built");
     default:
       return fail(AZB_ERR_UNSUPPORTED, "row length " + std::to_string(L) + " not built");
   }
@@ -191,6 +219,9 @@ struct azb_solver {
   size_t bytes = 0;
   long nmodes = 0;  // modes per component on the N-grid
   double loop_C = 0.2;  // CFL number of the device-driven loop
+  // tuning: column tile divisor per kernel family, line-pair divisor of the row kernel
+  int tsdiv[AZB_PROF_FAMILIES] = {1, 1, 1, 1, 1, 1};
+  int pdiv = 1;
   DevState* probe_host = nullptr;  // pinned mirror of the device state (non-blocking probes)
   hipEvent_t probe_ev = nullptr;
   // per-kernel-family hipEvent timing (off unless enabled)
@@ -262,7 +293,7 @@ int enqueue_rhs_T(azb_solver* s, const double2* u_in, const double2* u0, double2
     c.bps = 3;
     {
       ProfScope p(s, AZB_PROF_INV_AX0);
-      if ((rc = launch_col<double, T, T, +1, COL_PAD_IN>(M, c, 3L * S, st))) return rc;
+      if ((rc = launch_col<double, T, T, +1, COL_PAD_IN>(M, c, 3L * S, st, s->tsdiv[AZB_PROF_INV_AX0]))) return rc;
     }
     // K2: inverse along axis 1; X[c][i'][j][hp] -> Y[c][i'][j'][hp]
     c.in = s->X; c.out = s->Y; c.nq = 1;
@@ -272,14 +303,14 @@ int enqueue_rhs_T(azb_solver* s, const double2* u_in, const double2* u0, double2
     c.bps = 3 * M;
     {
       ProfScope p(s, AZB_PROF_INV_AX1);
-      if ((rc = launch_col<T, T, T, +1, COL_PAD_IN>(M, c, 3L * M * S, st))) return rc;
+      if ((rc = launch_col<T, T, T, +1, COL_PAD_IN>(M, c, 3L * M * S, st, s->tsdiv[AZB_PROF_INV_AX1]))) return rc;
     }
     // K3: last axis c2r, products, max, r2c; Y[c][line][hp] -> Z[m][line][hp]
     r.in = s->Y; r.out = s->Z; r.nlines = (long)M * M;
     r.in_ss = 3L * M * M * hp; r.out_ss = 6L * M * M * hp;
     {
       ProfScope p(s, AZB_PROF_ROW);
-      if ((rc = launch_row<T, 3>(M, r, S, st))) return rc;
+      if ((rc = launch_row<T, 3>(M, r, S, st, s->pdiv))) return rc;
     }
     // K4: forward along axis 1, M -> N rows; Z[m][i'][j'][hp] -> X[m][i'][j][hp]
     c.in = s->Z; c.out = s->X; c.nq = 1;
@@ -289,7 +320,7 @@ int enqueue_rhs_T(azb_solver* s, const double2* u_in, const double2* u0, double2
     c.bps = 6 * M;
     {
       ProfScope p(s, AZB_PROF_FWD_AX1);
-      if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 6L * M * S, st))) return rc;
+      if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 6L * M * S, st, s->tsdiv[AZB_PROF_FWD_AX1]))) return rc;
     }
     // K5: forward along axis 0; X[m][i'][j][hp] -> Y[m][i][j][hp] (B_hat on the N-grid)
     c.in = s->X; c.out = s->Y; c.nq = N;
@@ -299,7 +330,7 @@ int enqueue_rhs_T(azb_solver* s, const double2* u_in, const double2* u0, double2
     c.bps = 6;
     {
       ProfScope p(s, AZB_PROF_FWD_AX0);
-      if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 6L * S, st))) return rc;
+      if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 6L * S, st, s->tsdiv[AZB_PROF_FWD_AX0]))) return rc;
     }
   } else {
     // K1: inverse along axis 0; state [c][i][k] -> X[c][i'][hp]
@@ -310,14 +341,14 @@ int enqueue_rhs_T(azb_solver* s, const double2* u_in, const double2* u0, double2
     c.bps = 2;
     {
       ProfScope p(s, AZB_PROF_INV_AX0);
-      if ((rc = launch_col<double, T, T, +1, COL_PAD_IN>(M, c, 2L * S, st))) return rc;
+      if ((rc = launch_col<double, T, T, +1, COL_PAD_IN>(M, c, 2L * S, st, s->tsdiv[AZB_PROF_INV_AX0]))) return rc;
     }
     // K2: last axis; X[c][i'][hp] -> Z[m][i'][hp]
     r.in = s->X; r.out = s->Z; r.nlines = M;
     r.in_ss = 2L * M * hp; r.out_ss = 3L * M * hp;
     {
       ProfScope p(s, AZB_PROF_ROW);
-      if ((rc = launch_row<T, 2>(M, r, S, st))) return rc;
+      if ((rc = launch_row<T, 2>(M, r, S, st, s->pdiv))) return rc;
     }
     // K3: forward along axis 0; Z[m][i'][hp] -> Y[m][i][hp]
     c.in = s->Z; c.out = s->Y; c.nq = 1;
@@ -327,7 +358,7 @@ int enqueue_rhs_T(azb_solver* s, const double2* u_in, const double2* u0, double2
     c.bps = 3;
     {
       ProfScope p(s, AZB_PROF_FWD_AX0);
-      if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 3L * S, st))) return rc;
+      if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 3L * S, st, s->tsdiv[AZB_PROF_FWD_AX0]))) return rc;
     }
   }
   const double Md = std::pow((double)M, s->dim);
@@ -737,6 +768,77 @@ int azb_profile_reset(azb_solver* s) {
     s->prof_n[i] = 0;
   }
   return AZB_OK;
+}
+
+int azb_set_tuning(azb_solver* s, const int* tsdiv, int pdiv) {
+  if (!s || !tsdiv) return fail(AZB_ERR_ARG, "NULL argument");
+  for (int i = 0; i < AZB_PROF_FAMILIES; ++i) {
+    const int v = tsdiv[i];
+    if (v != 1 && v != 2 && v != 4) return fail(AZB_ERR_ARG, "tile divisors must be 1, 2 or 4");
+    s->tsdiv[i] = v;
+  }
+  if (pdiv != 1 && pdiv != 2) return fail(AZB_ERR_ARG, "row divisor must be 1 or 2");
+  const int P = row_p(s->M);
+  const long lines = s->dim == 3 ? (long)s->M * s->M : (long)s->M;
+  s->pdiv = (pdiv == 2 && P >= 2 && lines % P == 0) ? 2 : 1;
+  return AZB_OK;
+}
+
+int azb_get_tuning(const azb_solver* s, int* tsdiv, int* pdiv) {
+  if (!s || !tsdiv || !pdiv) return fail(AZB_ERR_ARG, "NULL argument");
+  for (int i = 0; i < AZB_PROF_FAMILIES; ++i) tsdiv[i] = s->tsdiv[i];
+  *pdiv = s->pdiv;
+  return AZB_OK;
+}
+
+int azb_autotune(azb_solver* s, const void* u_hat, int reps) {
+  if (!s || !u_hat) return fail(AZB_ERR_ARG, "NULL argument");
+  if (reps < 1) reps = 3;
+  const bool prof_was = s->prof;
+  const int cands[3] = {1, 2, 4};
+  double best[AZB_PROF_FAMILIES];
+  int pick[AZB_PROF_FAMILIES];
+  for (int f = 0; f < AZB_PROF_FAMILIES; ++f) {
+    best[f] = 1e300;
+    pick[f] = 1;
+  }
+  double best_row = 1e300;
+  int pick_row = 1;
+  int rc = AZB_OK;
+  for (int ci = 0; ci < 3; ++ci) {
+    const int c = cands[ci];
+    int td[AZB_PROF_FAMILIES];
+    for (int f = 0; f < AZB_PROF_FAMILIES; ++f) td[f] = c;
+    if ((rc = azb_set_tuning(s, td, c >= 2 ? 2 : 1))) break;
+    const bool row_variant_ok = (c == 1) || s->pdiv == 2;
+    // warm-up, then timed evaluations into the solver's own stage buffer
+    if ((rc = azb_dudt(s, u_hat, s->u1))) break;
+    if ((rc = azb_profile_reset(s))) break;
+    s->prof = true;
+    for (int r = 0; r < reps && rc == AZB_OK; ++r) rc = azb_dudt(s, u_hat, s->u1);
+    s->prof = false;
+    if (rc) break;
+    double ms[AZB_PROF_FAMILIES];
+    long n[AZB_PROF_FAMILIES];
+    if ((rc = azb_profile_read(s, ms, n, AZB_PROF_FAMILIES))) break;
+    for (int f = 0; f < AZB_PROF_FAMILIES; ++f) {
+      if (n[f] == 0) continue;
+      const double avg = ms[f] / n[f];
+      if (f == AZB_PROF_ROW) {
+        if (row_variant_ok && avg < best_row) {
+          best_row = avg;
+          pick_row = c >= 2 ? 2 : 1;
+        }
+      } else if (f != AZB_PROF_KSPACE && avg < best[f]) {
+        best[f] = avg;
+        pick[f] = c;
+      }
+    }
+  }
+  s->prof = prof_was;
+  (void)azb_profile_reset(s);
+  if (rc) return rc;
+  return azb_set_tuning(s, pick, pick_row);
 }
 
 int azb_synchronize(azb_solver* s) {

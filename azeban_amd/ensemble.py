@@ -57,6 +57,37 @@ class Ensemble:
             g.handle = h
         self.graph = None
 
+    def autotune(self, reps: int = 3, cache: str | None = None) -> list:
+        """Pick each group's launch shapes (column tile widths, row line pairs) by timing
+        them on that group's real batch; `cache` stores/reuses the choice per
+        (dim, N, precision, samples)."""
+        from .solver import _tuning_load, _tuning_save
+        from ._capi import PROF_FAMILIES
+
+        store = _tuning_load(cache) if cache else {}
+        out = []
+        for g in self.groups:
+            key = f"{g.dim}/{g.N}/{g.precision}/{g.S}"
+            if key in store:
+                rec = store[key]
+            else:
+                torch.cuda.synchronize()
+                check(self.lib.azb_autotune(g.handle, _vp(g.u), reps))
+                ts = (C.c_int * len(PROF_FAMILIES))()
+                pd = C.c_int()
+                check(self.lib.azb_get_tuning(g.handle, ts, C.byref(pd)))
+                rec = {"tsdiv": list(ts), "pdiv": pd.value}
+                store[key] = rec
+            ts = (C.c_int * len(PROF_FAMILIES))(*rec["tsdiv"])
+            check(self.lib.azb_set_tuning(g.handle, ts, rec["pdiv"]))
+            out.append(rec)
+        if cache:
+            _tuning_save(cache, store)
+        if self.graph is not None:  # launch shapes changed: capture again
+            self.lib.azb_graph_destroy(self.graph)
+            self.graph = None
+        return out
+
     def close(self):
         if self.graph is not None:
             self.lib.azb_graph_destroy(self.graph)

@@ -125,6 +125,33 @@ class IncompressibleEuler:
         check(load().azb_u_max(self.handle, C.byref(out)))
         return out.value
 
+    # launch-shape tuning: column tile divisors per kernel family, row line-pair divisor
+    def get_tuning(self):
+        ts = (C.c_int * len(_capi.PROF_FAMILIES))()
+        pd = C.c_int()
+        check(load().azb_get_tuning(self.handle, ts, C.byref(pd)))
+        return list(ts), pd.value
+
+    def set_tuning(self, tsdiv, pdiv: int) -> None:
+        ts = (C.c_int * len(_capi.PROF_FAMILIES))(*tsdiv)
+        check(load().azb_set_tuning(self.handle, ts, pdiv))
+
+    def autotune(self, u_hat: torch.Tensor, reps: int = 3, cache: str | None = None, nsamples: int = 1):
+        """Pick launch shapes by timing on the device; with `cache`, reuse a stored choice
+        keyed by (dim, N, precision, samples) or store the new one."""
+        key = f"{self.grid.dim}/{self.grid.N_phys}/{self.precision}/{nsamples}"
+        store = _tuning_load(cache) if cache else {}
+        if key in store:
+            self.set_tuning(store[key]["tsdiv"], store[key]["pdiv"])
+            return store[key]
+        check(load().azb_autotune(self.handle, _ptr(u_hat), reps))
+        ts, pd = self.get_tuning()
+        rec = {"tsdiv": ts, "pdiv": pd}
+        if cache:
+            store[key] = rec
+            _tuning_save(cache, store)
+        return rec
+
     # per-kernel-family hipEvent timing on the solver's stream
     def profile_enable(self, on: bool = True) -> None:
         check(load().azb_profile_enable(self.handle, 1 if on else 0))
@@ -143,6 +170,27 @@ class IncompressibleEuler:
         if tuple(u.shape) != self.grid.shape_fourier(self.n_vars()) or u.dtype != torch.complex128:
             raise ValueError(f"expected complex128 of shape {self.grid.shape_fourier(self.n_vars())}, "
                              f"got {u.dtype} {tuple(u.shape)}")
+
+
+def _tuning_load(path: str) -> dict:
+    import json
+    import os
+
+    if not os.path.exists(path):
+        return {}
+    with open(path) as f:
+        return json.load(f)
+
+
+def _tuning_save(path: str, store: dict) -> None:
+    import json
+    import os
+
+    os.makedirs(os.path.dirname(os.path.abspath(path)), exist_ok=True)
+    tmp = path + ".tmp"
+    with open(tmp, "w") as f:
+        json.dump(store, f, indent=1, sort_keys=True)
+    os.replace(tmp, path)
 
 
 class SSP_RK3:

@@ -19,6 +19,7 @@ import sys
 import time
 
 HBM_PEAK_GBS = 8000.0
+AUTOTUNE = True
 METRIC = "Mpts·RK-steps/s at 512^3 FP64 vs FP32-mixed; ensemble samples/s; p99 latency"
 
 
@@ -53,6 +54,10 @@ def run_precision(A, dim, N, u_hat0, precision, steps, warmup):
     visc = A.SmoothCutoff1D(0.05 / N, 1.0, 2 * math.pi * N / 4)
     eq = A.IncompressibleEuler(g, visc, precision)
     sim = A.Simulation(g, 0.2, A.SSP_RK3(g.shape_fourier(dim), eq), u_hat0)
+    tuning = None
+    if AUTOTUNE:
+        log(f"{precision}: autotune")
+        tuning = eq.autotune(sim.u(), reps=3)
     t_far = 1e9
     log(f"{precision}: warmup {warmup} steps")
     if warmup:
@@ -76,7 +81,7 @@ def run_precision(A, dim, N, u_hat0, precision, steps, warmup):
     sim.simulate_until(t_far, max_steps=steps, chunk=steps)
     fam = eq.profile_read()
     eq.profile_enable(False)
-    return {"eq": eq, "sim": sim, "wall_s": wall, "gpu_ms": gpu_ms, "families": fam}
+    return {"eq": eq, "sim": sim, "wall_s": wall, "gpu_ms": gpu_ms, "families": fam, "tuning": tuning}
 
 
 def cpu_baseline(dim: int, budget_s: float):
@@ -136,6 +141,9 @@ def run_ensemble(args):
             groups.append(Group(2, N, torch.stack(us).contiguous(), precision))
         init = [g.u.clone() for g in groups]
         ens = Ensemble(groups)
+        if AUTOTUNE:
+            log(f"ensemble {precision}: autotune")
+            ens.autotune(reps=2)
         log(f"ensemble {precision}: batched run")
         res = ens.run(t_end, steps_per_launch=args.ens_chunk)
         torch.cuda.synchronize()
@@ -210,7 +218,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calib-copy-gib", type=float, default=0.0,
                     help="run one device-to-device copy of this size first (counter calibration)")
+    ap.add_argument("--no-autotune", action="store_true")
     args = ap.parse_args()
+    global AUTOTUNE
+    AUTOTUNE = not args.no_autotune
 
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
@@ -297,6 +308,7 @@ def main():
                    "N": N, "dim": dim, "M": 3 * N // 2},
         "roofline": roofline,
         "kernels_fp64": kern64,
+        "tuning_fp64": res["fp64"]["tuning"],
     }
     if "fp32_mixed" in res:
         msm, kernm = summarize(res["fp32_mixed"], "fp32_mixed")

@@ -64,3 +64,29 @@ def test_graph_replays_repeatedly_match_eager(cuda):
         assert torch.equal(a.u, b.u)
     ea.close()
     eb.close()
+
+
+@pytest.mark.parametrize("dim,N", [(2, 1024), (2, 128), (3, 64)])
+def test_autotuned_shapes_give_identical_results(cuda, dim, N):
+    """Every launch shape the autotuner may pick computes each column / line pair
+    with the same arithmetic, so results are bit-identical to the defaults."""
+    import math
+
+    import azeban_amd as A
+
+    print(f"[case] autotune dim={dim} N={N}", flush=True)
+    u = initialize(I.fractional_brownian(dim, N, seed=3))
+    g = A.Grid(dim, N)
+    eq = A.IncompressibleEuler(g, A.SmoothCutoff1D(0.05 / N, 1.0, 2 * math.pi * N / 4))
+    ref = torch.empty_like(u)
+    eq.dudt(ref, u)
+    for ts, pd in [([2] * 6, 2), ([4] * 6, 2), ([4, 1, 2, 1, 4, 1], 1)]:
+        eq.set_tuning(ts, pd)
+        out = torch.empty_like(u)
+        eq.dudt(out, u)
+        assert torch.equal(out, ref), (ts, pd)
+    rec = eq.autotune(u, reps=2)
+    assert all(v in (1, 2, 4) for v in rec["tsdiv"]) and rec["pdiv"] in (1, 2)
+    out = torch.empty_like(u)
+    eq.dudt(out, u)
+    assert torch.equal(out, ref)
