@@ -11,6 +11,7 @@ from .solver import (  # noqa: F401
     SmoothCutoff1D,
     SSP_RK3,
     axpby_,
+    copy_from_padded,
     copy_to_padded,
     energy_spectrum,
     fft_c2r,
@@ -19,4 +20,6 @@ from .solver import (  # noqa: F401
     leray_,
     make_smooth_cutoff_1d,
     max_norm,
+    scale_,
+    to_physical,
 )

@@ -391,10 +391,13 @@ namespace azb {
 //   MAP_PAIR: a thread owns the butterfly pair (j, L/R - j) of one sequence (with
 //             0 paired with L/(2R)), so every output position and its mirror L - pos
 //             sit in the same thread after the last pass
+//   MAP_SEQ1: MAP_SEQ with G = 1 whatever the transform's G: consecutive threads own
+//             consecutive butterflies of one sequence (a radix-L/64 pass puts each
+//             sequence in exactly one wave, butterfly j in lane j)
 // Any mapping can be used by any pass: the LDS exchange writes with the mapping of
 // the pass that produced the data and reads with the mapping of the next pass.
 // ---------------------------------------------------------------------------
-enum { MAP_STD = 0, MAP_SEQ = 1, MAP_PAIR = 2 };
+enum { MAP_STD = 0, MAP_SEQ = 1, MAP_PAIR = 2, MAP_SEQ1 = 3 };
 
 template <int MAP, int TS, int NT, int L, int R, int G>
 __device__ __forceinline__ void bmap(int t, int b, int& s, int& j) {
@@ -402,11 +405,12 @@ __device__ __forceinline__ void bmap(int t, int b, int& s, int& j) {
     const int g = t + NT * b;
     s = g % TS;
     j = g / TS;
-  } else if (MAP == MAP_SEQ) {
-    const int q = b / G, gg = b % G;
+  } else if (MAP == MAP_SEQ || MAP == MAP_SEQ1) {
+    constexpr int GG = MAP == MAP_SEQ1 ? 1 : G;
+    const int q = b / GG, gg = b % GG;
     const int u = t + NT * q;
     j = u % (L / R);
-    s = (u / (L / R)) * G + gg;
+    s = (u / (L / R)) * GG + gg;
   } else {
     constexpr int H = L / (2 * R);
     const int q = b / 2, e = b % 2;
@@ -426,23 +430,26 @@ struct PlanAt<R0, Rs...> {
 
 // LDS layout of the exchange.  SWZ = false: element (s, pos) at pos*TS + s.
 // SWZ = true: sequence-major with one pad slot per 256-byte bank row and a
-// sequence stride of 1 modulo the row, so lane strides of 1, TS or R*TS in
-// either index stay spread over the banks.
-template <typename T, int L, int TS, bool SWZ>
+// sequence stride of SOFF modulo the row, so lane strides of 1, TS or R*TS in
+// either index stay spread over the banks (SOFF = 1 suits six interleaved
+// sequences; three interleaved sequences with consecutive positions want the
+// three runs of five or six slots apart, SOFF = 11).
+template <typename T, int L, int TS, bool SWZ, int SOFF = 1>
 struct ExLayout {
   static constexpr int PADE = 256 / (2 * (int)sizeof(T));
-  static constexpr int LP = ((L + L / PADE + PADE - 1) / PADE) * PADE + 1;
+  static constexpr int LP = ((L + L / PADE + PADE - 1) / PADE) * PADE + SOFF;
   static constexpr int elems = SWZ ? TS * LP : TS * L;
   __device__ __forceinline__ static int at(int s, int pos) {
     return SWZ ? s * LP + pos + pos / PADE : pos * TS + s;
   }
 };
 
-template <typename T, int L, int TS, int NT, int DIR, bool SPLIT, int G, int MAPF, int MAPL, bool SWZ = false>
+template <typename T, int L, int TS, int NT, int DIR, bool SPLIT, int G, int MAPF, int MAPL, bool SWZ = false,
+          int SOFF = 1>
 struct BlockFFT2 {
   static constexpr int E = TS * L / NT;
   static_assert(TS * L % NT == 0, "TS*L must be a multiple of NT");
-  using LY = ExLayout<T, L, TS, SWZ>;
+  using LY = ExLayout<T, L, TS, SWZ, SOFF>;
   // LDS words (of T) the exchange needs
   static constexpr int lds_words = SPLIT ? LY::elems : 2 * LY::elems;
 
@@ -475,8 +482,24 @@ struct BlockFFT2 {
         bmap<MAP, TS, NT, L, R, G>(t, b, s, j);
         const int k = j % Ns;
         constexpr int step = L / (Ns * R);
+        if (sizeof(T) == 8) {
+          // FP64: one table load per butterfly, the other powers by repeated
+          // multiplication (w^(r+1) = w^r w, below 22 ulp at R = 24).  Each saved
+          // load is a full vector-memory instruction through the texture path; the
+          // chain costs four FMA-class operations and keeps two powers live.
+          const cx<T> w1 = twiddle(tw, k * step);
+          cx<T> wr = w1;
+          w[1] = cmul(w[1], w1);
 #pragma unroll
-        for (int r = 1; r < R; ++r) w[r] = cmul(w[r], twiddle(tw, r * k * step));
+          for (int r = 2; r < R; ++r) {
+            wr = cmul(wr, w1);
+            w[r] = cmul(w[r], wr);
+          }
+        } else {
+          // FP32 loads every power: a chain's error would rival the transform's own
+#pragma unroll
+          for (int r = 1; r < R; ++r) w[r] = cmul(w[r], twiddle(tw, r * k * step));
+        }
       }
       DFT<R, DIR, T>::run(w);
     }

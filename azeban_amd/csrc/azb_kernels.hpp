@@ -681,6 +681,32 @@ __global__ void pad_kernel(const double2* in, double2* out, int dim, int N, int 
   out[idx] = r;
 }
 
+// truncation of one component M -> N (copy_from_padded_functions.hpp:11-27,
+// copy_from_padded.cpp:28-86): full axes read i < N/2+1 from i and the rest from
+// i + M - N; the compact last axis reads k from k.
+__global__ void unpad_kernel(const double2* in, double2* out, int dim, int M, int N) {
+  const int nh = N / 2 + 1, mh = M / 2 + 1;
+  const long total = dim == 2 ? (long)N * nh : (long)N * N * nh;
+  const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= total) return;
+  const int k = (int)(idx % nh);
+  const long rest = idx / nh;
+  auto src_full = [&](int i) -> long { return i < N / 2 + 1 ? i : i + M - N; };
+  if (dim == 2) {
+    out[idx] = in[src_full((int)rest) * mh + k];
+  } else {
+    const long j = src_full((int)(rest % N));
+    const long i = src_full((int)(rest / N));
+    out[idx] = in[(i * M + j) * mh + k];
+  }
+}
+
+// x <- a x over a flat real array (scale.hpp)
+__global__ void scale_kernel(double a, double* x, long n) {
+  const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] *= a;
+}
+
 // y <- a x + b y over a flat complex array (axpby.hpp:41-46)
 __global__ void axpby_kernel(double a, const double2* x, double b, double2* y, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -956,6 +982,187 @@ __global__ __launch_bounds__(P * L / 8, (sizeof(T) == 8 ? 2 : 4)) void euler_row
   if (t == 0) {
     double mx = 0.0;
     for (int w = 0; w < (NT + 63) / 64; ++w) mx = fmax(mx, red[w]);
+    atomicMax(&st->umax_bins[a.slot][blockIdx.x & (kBins - 1)],
+              (unsigned long long)__double_as_longlong(mx));
+  }
+}
+
+
+// ---------------------------------------------------------------------------
+// 3D row kernel, 12 values per thread.  Per workgroup: P line pairs (a, b),
+// TS = 3P packed sequences, NT = P*L/4 threads.
+//   inverse  : packed c2r of (u1a + i u2a, u3a + i u3b, u1b + i u2b), run once.  Its
+//              first pass (radix 12, L/12 butterflies per sequence, consecutive
+//              butterflies in consecutive lanes) loads straight from HBM with
+//              contiguous lanes; its last pass (radix 4, MAP_SEQ) leaves u1, u2, u3 of
+//              both lines at the same 4 positions in every thread, where they stay
+//              in registers for both product rounds;
+//   products : round 0 -> (B11, B21, B22), round 1 -> (B31, B32, B33), packed
+//              (line a) + i (line b), feeding the forward transform's MAP_SEQ first
+//              pass at exactly those positions;
+//   forward  : last pass radix 12 with butterfly j of a sequence in lane j of its
+//              wave, so the butterfly holding the mirrored positions L - pos
+//              (butterfly L/12 - j) is another lane of the same wave: the four mirror
+//              values a thread needs for separation come over lane permutes, and
+//              the k <= N/2 outputs leave as contiguous 16-byte stores.
+// ---------------------------------------------------------------------------
+template <int L>
+struct Row4Plan;
+template <> struct Row4Plan<768> { using Inv = Plan<12, 4, 4, 4>; using Fwd = Plan<4, 4, 4, 12>; };
+template <> struct Row4Plan<384> { using Inv = Plan<12, 2, 4, 4>; using Fwd = Plan<4, 4, 2, 12>; };
+template <> struct Row4Plan<192> { using Inv = Plan<12, 4, 4>; using Fwd = Plan<4, 4, 12>; };
+template <> struct Row4Plan<96> { using Inv = Plan<12, 2, 4>; using Fwd = Plan<4, 2, 12>; };
+
+template <typename T, int L, int P>
+struct Row4Cfg {
+  static constexpr int TS = 3 * P;
+  static constexpr int NT = P * L / 4;
+  static constexpr int E = 12;
+  static constexpr int N = 2 * L / 3;
+  static constexpr int NB = L / 12;  // butterflies per sequence in the radix-12 passes
+  static constexpr int SOFF = 11;
+  static constexpr int XWORDS = 2 * ExLayout<T, L, TS, true, SOFF>::elems;
+};
+
+template <typename T>
+using vec2_t = T __attribute__((ext_vector_type(2)));
+
+template <typename T, int L, int P>
+__global__ __launch_bounds__(P * L / 4, (sizeof(T) == 8 ? 3 : 4)) void euler_row4_kernel(RowArgs a) {
+  using C = Row4Cfg<T, L, P>;
+  using PL = Row4Plan<L>;
+  constexpr int NT = C::NT, E = C::E, N = C::N, TS = C::TS, NB = C::NB;
+  static_assert(64 % NB == 0 && NT % 64 == 0, "a wave must hold whole sequences in the radix-12 passes");
+  using FI = BlockFFT2<T, L, TS, NT, +1, false, 3, MAP_SEQ1, MAP_SEQ, true, C::SOFF>;
+  using FF = BlockFFT2<T, L, TS, NT, -1, false, 3, MAP_SEQ, MAP_SEQ1, true, C::SOFF>;
+  static_assert(FI::E == E && FF::E == E && FI::lds_words == C::XWORDS, "exchange size");
+  static_assert(first_radix(typename PL::Inv{}) == 12 && last_radix(typename PL::Inv{}) == 4 &&
+                    first_radix(typename PL::Fwd{}) == 4 && last_radix(typename PL::Fwd{}) == 12,
+                "plans must start/end on the radices the mappings assume");
+  __shared__ T X[C::XWORDS];
+  __shared__ double red[NT / 64];
+  const long bps = a.nlines / (2 * P);  // workgroups per sample
+  const long sample = blockIdx.x / bps;
+  DevState* st = a.st + sample;
+  if (st->done) return;
+
+  const int t = threadIdx.x;
+  const long line0 = (blockIdx.x % bps) * (2 * P);
+  const cx<T>* __restrict__ in = (const cx<T>*)a.in + sample * a.in_ss;
+  cx<T>* __restrict__ out = (cx<T>*)a.out + sample * a.out_ss;
+  const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
+  const T nrm = T(a.norm);
+
+  // packed inverse input: sequence s of pair s/3, type s%3
+  auto ld_inv = [&](int s, int pos) -> cx<T> {
+    const int pair = s / 3, type = s % 3;
+    const long la = line0 + 2 * pair, lb = la + 1;
+    int kk;
+    int mode;  // 1: direct, 2: mirrored, 3: k = 0
+    if (pos == 0) {
+      kk = 0; mode = 3;
+    } else if (pos <= N / 2) {
+      kk = pos; mode = 1;
+    } else if (pos >= L - N / 2) {
+      kk = L - pos; mode = 2;
+    } else {
+      return mk(T(0), T(0));
+    }
+    long ix, iy;
+    if (type == 0) {
+      ix = (0 * a.nlines + la) * a.ls; iy = (1 * a.nlines + la) * a.ls;
+    } else if (type == 1) {
+      ix = (2 * a.nlines + la) * a.ls; iy = (2 * a.nlines + lb) * a.ls;
+    } else {
+      ix = (0 * a.nlines + lb) * a.ls; iy = (1 * a.nlines + lb) * a.ls;
+    }
+    const cx<T> X0 = in[ix + kk], Y0 = in[iy + kk];
+    if (mode == 1) return mk(X0.x - Y0.y, X0.y + Y0.x);
+    if (mode == 2) return mk(X0.x + Y0.y, Y0.x - X0.y);
+    return mk(X0.x, Y0.x);
+  };
+
+  cx<T> v[E];
+  FI::template load_dft<12>(v, t, ld_inv);
+  FI::run_rest(v, t, X, tw, typename PL::Inv{});
+  // u[g*4 + r]: packed sequence type g of this thread's pair at its 4 positions
+  cx<T> u[E];
+  double lmax = 0.0;
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    u[e] = v[e];
+    lmax = fmax(lmax, (double)fmax(fabs(v[e].x), fabs(v[e].y)));
+  }
+
+#pragma unroll 1
+  for (int round = 0; round < 2; ++round) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      // p0 = (u1a, u2a), p1 = (u3a, u3b), p2 = (u1b, u2b)
+      const cx<T> p0 = u[r], p1 = u[4 + r], p2 = u[8 + r];
+      const T a1 = nrm * p0.x, a2 = nrm * p0.y, b1 = nrm * p2.x, b2 = nrm * p2.y;
+      if (round == 0) {
+        v[r] = mk(a1 * a1, b1 * b1);      // B11
+        v[4 + r] = mk(a2 * a1, b2 * b1);  // B21
+        v[8 + r] = mk(a2 * a2, b2 * b2);  // B22
+      } else {
+        const T a3 = nrm * p1.x, b3 = nrm * p1.y;
+        v[r] = mk(a3 * a1, b3 * b1);      // B31
+        v[4 + r] = mk(a3 * a2, b3 * b2);  // B32
+        v[8 + r] = mk(a3 * a3, b3 * b3);  // B33
+      }
+    }
+    FF::run(v, t, X, tw, typename PL::Fwd{});
+    // separation.  This thread holds Z[j + NB r], r < 12, of sequence s.  The mirror
+    // of position j + NB r is (NB - j) + NB (11 - r): index 11 - r of the lane that
+    // holds butterfly NB - j (for j = 0: index 12 - r of this lane).  Outputs are
+    // the positions k <= N/2 = 4 NB: r < 4, and r = 4 for j = 0.
+    {
+      int tt = t;
+      asm volatile("" : "+v"(tt));
+      const int s = tt / NB, j = tt % NB;
+      const int pair = s / 3, m = round * 3 + s % 3;
+      const long la = line0 + 2 * pair, lb = la + 1;
+      vec2_t<T>* oa = (vec2_t<T>*)(out + ((long)m * a.nlines + la) * a.ls);
+      vec2_t<T>* ob = (vec2_t<T>*)(out + ((long)m * a.nlines + lb) * a.ls);
+      const int lane = tt & 63;
+      const int mlane = lane - j + (NB - j) % NB;
+      cx<T> f[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        f[q].x = __shfl(v[11 - q].x, mlane);
+        f[q].y = __shfl(v[11 - q].y, mlane);
+      }
+      auto emit = [&](int pos, const cx<T>& zk, const cx<T>& zm) {
+        vec2_t<T> va, vb;
+        va.x = T(0.5) * (zk.x + zm.x);
+        va.y = T(0.5) * (zk.y - zm.y);
+        vb.x = T(0.5) * (zk.y + zm.y);
+        vb.y = T(-0.5) * (zk.x - zm.x);
+        __builtin_nontemporal_store(va, oa + pos);
+        __builtin_nontemporal_store(vb, ob + pos);
+      };
+      const bool j0 = j == 0;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        cx<T> zm = f[r];
+        if (r == 0) {
+          if (j0) zm = v[0];
+        } else {
+          if (j0) zm = f[r - 1];
+        }
+        emit(j + NB * r, v[r], zm);
+      }
+      if (j0) emit(4 * NB, v[4], f[3]);
+    }
+  }
+
+  lmax = wave_max<T>(lmax);
+  if ((t & 63) == 0) red[t >> 6] = lmax;
+  __syncthreads();
+  if (t == 0) {
+    double mx = 0.0;
+    for (int w = 0; w < NT / 64; ++w) mx = fmax(mx, red[w]);
     atomicMax(&st->umax_bins[a.slot][blockIdx.x & (kBins - 1)],
               (unsigned long long)__double_as_longlong(mx));
   }

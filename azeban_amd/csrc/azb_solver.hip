@@ -132,16 +132,27 @@ void launch_row3_P(RowArgs a, long nsamples, hipStream_t st) {
   hipLaunchKernelGGL((euler_row3_kernel<T, L, P>), dim3((unsigned)blocks), dim3(P * L / 8), 0, st, a);
 }
 
+template <typename T, int L, int P>
+void launch_row4_P(RowArgs a, long nsamples, hipStream_t st) {
+  const long blocks = a.nlines / (2 * P) * nsamples;
+  hipLaunchKernelGGL((euler_row4_kernel<T, L, P>), dim3((unsigned)blocks), dim3(Row4Cfg<T, L, P>::NT), 0, st, a);
+}
+
 // pdiv in {1, 2}: P0 = 1536/L line pairs per workgroup, or half of that
+// (3D: 3 and 4 select the 24-values-per-thread kernel with P0 / P1)
 template <typename T, int L, int DIM>
 void launch_row_L(RowArgs a, long nsamples, hipStream_t st, int pdiv) {
   constexpr int P0 = (1536 / L) < 1 ? 1 : 1536 / L;
   constexpr int P1 = P0 / 2 >= 1 ? P0 / 2 : 1;
   if constexpr (DIM == 3) {
-    if (pdiv >= 2)
+    if (pdiv == 4)
       launch_row3_P<T, L, P1>(a, nsamples, st);
-    else
+    else if (pdiv == 3)
       launch_row3_P<T, L, P0>(a, nsamples, st);
+    else if (pdiv == 2)
+      launch_row4_P<T, L, P1>(a, nsamples, st);
+    else
+      launch_row4_P<T, L, P0>(a, nsamples, st);
   } else {
     if (pdiv >= 2)
       launch_row_P<T, L, P1, DIM>(a, nsamples, st);
@@ -153,7 +164,7 @@ void launch_row_L(RowArgs a, long nsamples, hipStream_t st, int pdiv) {
 template <typename T, int DIM>
 int launch_row(int L, RowArgs a, long nsamples, hipStream_t st, int pdiv = 1) {
   int P = row_p(L);
-  if (pdiv >= 2 && P >= 2) P /= 2;
+  if ((pdiv == 2 || pdiv == 4) && P >= 2) P /= 2;
   if (a.nlines % (2 * P) != 0) return fail(AZB_ERR_UNSUPPORTED, "line count not a multiple of the line-pair tile");
   if (DIM == 3 && L == 1536) return fail(AZB_ERR_UNSUPPORTED, "3D row length 1536 not built");
   switch (L) {
@@ -436,6 +447,8 @@ int azb_create_batch(int dim, int N, int M, int n_vars, int precision, azb_sv_pa
   s->nvars = n_vars;
   s->prec = precision;
   s->visc = visc;
+  // 3D: the 12-values-per-thread row kernel with 192-thread workgroups
+  s->pdiv = dim == 3 ? 2 : 1;
   s->tsize = precision == AZB_FP64 ? 8 : 4;
   const size_t cs = 2 * s->tsize;  // complex at working precision
   const long nh = s->nh;
@@ -777,10 +790,11 @@ int azb_set_tuning(azb_solver* s, const int* tsdiv, int pdiv) {
     if (v != 1 && v != 2 && v != 4) return fail(AZB_ERR_ARG, "tile divisors must be 1, 2 or 4");
     s->tsdiv[i] = v;
   }
-  if (pdiv != 1 && pdiv != 2) return fail(AZB_ERR_ARG, "row divisor must be 1 or 2");
+  if (pdiv < 1 || pdiv > (s->dim == 3 ? 4 : 2)) return fail(AZB_ERR_ARG, "row variant out of range");
   const int P = row_p(s->M);
   const long lines = s->dim == 3 ? (long)s->M * s->M : (long)s->M;
-  s->pdiv = (pdiv == 2 && P >= 2 && lines % P == 0) ? 2 : 1;
+  const bool half = pdiv == 2 || pdiv == 4;
+  s->pdiv = (half && !(P >= 2 && lines % P == 0)) ? pdiv - 1 : pdiv;
   return AZB_OK;
 }
 
@@ -958,6 +972,27 @@ int azb_copy_to_padded(int dim, int N, int M, const void* in, void* out, void* s
   const long total = dim == 2 ? (long)M * mh : (long)M * M * mh;
   hipLaunchKernelGGL(pad_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
                      (hipStream_t)stream, (const double2*)in, (double2*)out, dim, N, M);
+  HIPCHK(hipGetLastError());
+  return AZB_OK;
+}
+
+int azb_copy_from_padded(int dim, int M, int N, const void* in, void* out, void* stream) {
+  if (!in || !out) return fail(AZB_ERR_ARG, "NULL argument");
+  if ((dim != 2 && dim != 3) || N < 2 || M < N) return fail(AZB_ERR_ARG, "bad dim or sizes");
+  const long nh = N / 2 + 1;
+  const long total = dim == 2 ? (long)N * nh : (long)N * N * nh;
+  hipLaunchKernelGGL(unpad_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, (const double2*)in, (double2*)out, dim, M, N);
+  HIPCHK(hipGetLastError());
+  return AZB_OK;
+}
+
+int azb_scale(double a, void* x, long n, void* stream) {
+  if (!x) return fail(AZB_ERR_ARG, "NULL argument");
+  if (n < 0) return fail(AZB_ERR_ARG, "negative length");
+  if (n == 0) return AZB_OK;
+  hipLaunchKernelGGL(scale_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream, a,
+                     (double*)x, n);
   HIPCHK(hipGetLastError());
   return AZB_OK;
 }

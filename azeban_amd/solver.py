@@ -322,6 +322,44 @@ def copy_to_padded(src: torch.Tensor, M: int) -> torch.Tensor:
     return out
 
 
+def copy_from_padded(src: torch.Tensor, N: int) -> torch.Tensor:
+    """One component, M -> N truncation (the inverse of copy_to_padded)."""
+    dim = src.dim()
+    M = src.shape[0]
+    shape = (N,) * (dim - 1) + (N // 2 + 1,)
+    out = torch.empty(shape, dtype=torch.complex128, device=src.device)
+    src = src.contiguous()
+    check(load().azb_copy_from_padded(dim, M, N, _ptr(src), _ptr(out), _stream()))
+    return out
+
+
+def scale_(a: float, x: torch.Tensor) -> torch.Tensor:
+    """x <- a x in place over a float64 or complex128 tensor."""
+    if not x.is_contiguous() or x.dtype not in (torch.float64, torch.complex128):
+        raise ValueError("scale_ needs a contiguous float64 or complex128 tensor")
+    n = x.numel() * (2 if x.is_complex() else 1)
+    check(load().azb_scale(a, _ptr(x), n, _stream()))
+    return x
+
+
+def to_physical(u_hat: torch.Tensor, N_out: int | None = None) -> torch.Tensor:
+    """Velocity in physical space as azeban writes it (run_from_config.cpp:93-98): u_hat
+    scaled by 1/N^d, then the unnormalised inverse transform.  With N_out < N the
+    spectrum is first truncated to N_out modes per axis, as the sample writer's
+    downsampled output does (netcdf_sample_writer.cpp:143-163); the scale stays 1/N^d."""
+    dim = u_hat.dim() - 1
+    N = u_hat.shape[1]
+    n = N if N_out is None else N_out
+    if n > N:
+        raise ValueError("N_out must not exceed the grid size")
+    if n == N:
+        tmp = u_hat.contiguous().clone()
+    else:
+        tmp = torch.stack([copy_from_padded(u_hat[c], n) for c in range(u_hat.shape[0])])
+    scale_(1.0 / float(N) ** dim, tmp)
+    return fft_c2r(tmp, n)
+
+
 def axpby_(a: float, x: torch.Tensor, b: float, y: torch.Tensor) -> torch.Tensor:
     check(load().azb_axpby(a, _ptr(x), b, _ptr(y), x.numel(), _stream()))
     return y

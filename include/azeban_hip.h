@@ -172,6 +172,17 @@ int azb_fft_c2r(int dim, int n, int howmany, int precision, void* in, void* out,
  *  include/azeban/operations/copy_to_padded_functions.hpp:11-63). */
 int azb_copy_to_padded(int dim, int N, int M, const void* in, void* out, void* stream);
 
+/* One component M -> N truncation, the inverse of azb_copy_to_padded: copy_from_padded
+ * (src/azeban/operations/copy_from_padded.cpp:28-86,
+ *  include/azeban/operations/copy_from_padded_functions.hpp:11-27).  Used to write
+ * downsampled output (src/azeban/io/netcdf_sample_writer.cpp:143-163). */
+int azb_copy_from_padded(int dim, int M, int N, const void* in, void* out, void* stream);
+
+/* x <- a x over n doubles, scale (include/azeban/operations/scale.hpp); the physical
+ * output path scales u_hat by 1/N^dim before the inverse transform
+ * (src/azeban/run_from_config.cpp:93-98). */
+int azb_scale(double a, void* x, long n, void* stream);
+
 /* y <- a x + b y over n complex doubles, axpby (include/azeban/operations/axpby.hpp:28-56). */
 int azb_axpby(double a, const void* x, double b, void* y, long n, void* stream);
 

@@ -161,6 +161,23 @@ def copy_to_padded(src: np.ndarray, dst_shape, pad_value=0.0) -> np.ndarray:
     return dst
 
 
+
+def copy_from_padded(src: np.ndarray, dst_shape) -> np.ndarray:
+    """One component, the truncation inverse of copy_to_padded
+    (include/azeban/operations/copy_from_padded_functions.hpp:11-27, unpad_full_dim /
+    unpad_compact_dim; src/azeban/operations/copy_from_padded.cpp:28-86): full axes read
+    i < N/2+1 from i and i >= N/2+1 from i + M - N; the compact last axis reads k from k."""
+    d = src.ndim
+    idx = []
+    for ax in range(d):
+        n_dst = dst_shape[ax]
+        ii = np.arange(n_dst)
+        if ax == d - 1:
+            idx.append(ii)
+        else:
+            idx.append(np.where(ii < n_dst // 2 + 1, ii, ii + src.shape[ax] - n_dst))
+    return src[np.ix_(*idx)].astype(np.complex128)
+
 # ---------------------------------------------------------------------------
 # FFT (src/azeban/operations/fftwfft.cpp:258-350): unnormalised, batched over
 # the leading component axis, rank = dim.

@@ -123,6 +123,26 @@ def test_copy_to_padded_structural(dim, N, M):
         assert dst[idx] == m.get(key, 0)
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_copy_from_padded_round_trip_and_signed_modes(dim):
+    """test/azeban/operations/copy_from_padded.cpp:21-66: unpadding a padded array gives
+    the source back exactly, for M in {N, 3N/2, 2N}; and each kept entry is the padded
+    entry at the same signed wavenumber (independent of copy_to_padded)."""
+    rng = np.random.default_rng(5)
+    for N in (8, 16) if dim == 3 else (8, 16, 32):
+        for M in (N, 3 * N // 2, 2 * N):
+            src_shape = (N,) * (dim - 1) + (N // 2 + 1,)
+            dst_shape = (M,) * (dim - 1) + (M // 2 + 1,)
+            src = rng.uniform(-1, 1, src_shape) + 1j * rng.uniform(-1, 1, src_shape)
+            assert np.array_equal(O.copy_from_padded(O.copy_to_padded(src, dst_shape), src_shape), src)
+            big = rng.uniform(-1, 1, dst_shape) + 1j * rng.uniform(-1, 1, dst_shape)
+            small = O.copy_from_padded(big, src_shape)
+            by_k = {tuple(to_real_k(i, M) for i in idx[:-1]) + (idx[-1],): big[idx]
+                    for idx in np.ndindex(*dst_shape)}
+            for idx in np.ndindex(*src_shape):
+                assert small[idx] == by_k[tuple(to_real_k(i, N) for i in idx[:-1]) + (idx[-1],)]
+
+
 # --------------------------------------------------------------------------- norm / SV
 def test_max_norm_known_answer():
     n = 1000 * 1000
