@@ -252,10 +252,16 @@ struct BlockFFT {
   static constexpr int E = TS * L / NT;
   static_assert(TS * L % NT == 0, "TS*L must be a multiple of NT");
 
-  // twiddle W_L^{e} with direction applied
-  __device__ __forceinline__ static cx<T> twiddle(const cx<T>* __restrict__ tw, int e) {
-    cx<T> w = tw[e];
-    if (DIR > 0) w.y = -w.y;
+  // twiddle W_L^{e} with direction applied; the table pointer is in the global
+  // address space: global (not flat) loads
+  // with a scalar base and a 32-bit offset, and no flat-load entries in the LDS
+  // wait counter
+  using twptr = const __attribute__((address_space(1))) cx<T>*;
+
+  __device__ __forceinline__ static cx<T> twiddle(twptr tw, unsigned e) {
+    cx<T> w;
+    w.x = tw[e].x;
+    w.y = DIR > 0 ? -tw[e].y : tw[e].y;
     return w;
   }
 
@@ -271,7 +277,7 @@ struct BlockFFT {
         const int k = j % Ns;
         constexpr int step = L / (Ns * R);
 #pragma unroll
-        for (int r = 1; r < R; ++r) w[r] = cmul(w[r], twiddle(tw, r * k * step));
+        for (int r = 1; r < R; ++r) w[r] = cmul(w[r], twiddle((twptr)tw, r * k * step));
       }
       DFT<R, DIR, T>::run(w);
     }
@@ -399,25 +405,27 @@ namespace azb {
 // ---------------------------------------------------------------------------
 enum { MAP_STD = 0, MAP_SEQ = 1, MAP_PAIR = 2, MAP_SEQ1 = 3 };
 
+// Index arithmetic is unsigned throughout: signed division and modulo by a power of
+// two cost a sign fix-up each, and these run once per element of every exchange.
 template <int MAP, int TS, int NT, int L, int R, int G>
-__device__ __forceinline__ void bmap(int t, int b, int& s, int& j) {
+__device__ __forceinline__ void bmap(unsigned t, unsigned b, unsigned& s, unsigned& j) {
   if (MAP == MAP_STD) {
-    const int g = t + NT * b;
+    const unsigned g = t + NT * b;
     s = g % TS;
     j = g / TS;
   } else if (MAP == MAP_SEQ || MAP == MAP_SEQ1) {
-    constexpr int GG = MAP == MAP_SEQ1 ? 1 : G;
-    const int q = b / GG, gg = b % GG;
-    const int u = t + NT * q;
-    j = u % (L / R);
-    s = (u / (L / R)) * GG + gg;
+    constexpr unsigned GG = MAP == MAP_SEQ1 ? 1 : G;
+    const unsigned q = b / GG, gg = b % GG;
+    const unsigned u = t + NT * q;
+    j = u % (unsigned)(L / R);
+    s = (u / (unsigned)(L / R)) * GG + gg;
   } else {
-    constexpr int H = L / (2 * R);
-    const int q = b / 2, e = b % 2;
-    const int u = t + NT * q;
+    constexpr unsigned H = L / (2 * R);
+    const unsigned q = b / 2, e = b % 2;
+    const unsigned u = t + NT * q;
     s = u / H;
-    const int p = u % H;
-    j = e == 0 ? p : (p == 0 ? H : L / R - p);
+    const unsigned p = u % H;
+    j = e == 0 ? p : (p == 0 ? H : (unsigned)(L / R) - p);
   }
 }
 
@@ -439,9 +447,12 @@ struct ExLayout {
   static constexpr int PADE = 256 / (2 * (int)sizeof(T));
   static constexpr int LP = ((L + L / PADE + PADE - 1) / PADE) * PADE + SOFF;
   static constexpr int elems = SWZ ? TS * LP : TS * L;
-  __device__ __forceinline__ static int at(int s, int pos) {
-    return SWZ ? s * LP + pos + pos / PADE : pos * TS + s;
+  __device__ __forceinline__ static unsigned at(unsigned s, unsigned pos) {
+    return SWZ ? s * (unsigned)LP + pos + pos / (unsigned)PADE : pos * (unsigned)TS + s;
   }
+  // at(s, pos + D) - at(s, pos) when that does not depend on pos (D a multiple of PADE)
+  static constexpr bool linear(int D) { return !SWZ || D % PADE == 0; }
+  static constexpr unsigned delta(int D) { return SWZ ? D + D / PADE : D * TS; }
 };
 
 template <typename T, int L, int TS, int NT, int DIR, bool SPLIT, int G, int MAPF, int MAPL, bool SWZ = false,
@@ -458,9 +469,15 @@ struct BlockFFT2 {
     return P == 0 ? MAPF : (P == NP - 1 ? MAPL : MAP_STD);
   }
 
-  __device__ __forceinline__ static cx<T> twiddle(const cx<T>* __restrict__ tw, int e) {
-    cx<T> w = tw[e];
-    if (DIR > 0) w.y = -w.y;
+  // twiddle table pointer in the global address space: global (not flat) loads
+  // with a scalar base and a 32-bit offset, and no flat-load entries in the LDS
+  // wait counter
+  using twptr = const __attribute__((address_space(1))) cx<T>*;
+
+  __device__ __forceinline__ static cx<T> twiddle(twptr tw, unsigned e) {
+    cx<T> w;
+    w.x = tw[e].x;
+    w.y = DIR > 0 ? -tw[e].y : tw[e].y;
     return w;
   }
 
@@ -470,18 +487,18 @@ struct BlockFFT2 {
     // keep this pass's twiddle loads inside the pass: without the opaque copy the
     // compiler hoists the (data-independent) loads of every pass of every transform
     // in the kernel to its top and holds them all in registers
-    const cx<T>* tw = tw_in;
+    twptr tw = (twptr)tw_in;
     asm volatile("" : "+s"(tw));
-    int t = t_in;
+    unsigned t = t_in;
     asm volatile("" : "+v"(t));
 #pragma unroll
     for (int b = 0; b < E / R; ++b) {
       cx<T>* w = v + b * R;
       if (Ns > 1) {
-        int s, j;
+        unsigned s, j;
         bmap<MAP, TS, NT, L, R, G>(t, b, s, j);
-        const int k = j % Ns;
-        constexpr int step = L / (Ns * R);
+        const unsigned k = j % Ns;
+        constexpr unsigned step = L / (Ns * R);
         if (sizeof(T) == 8) {
           // FP64: one table load per butterfly, the other powers by repeated
           // multiplication (w^(r+1) = w^r w, below 22 ulp at R = 24).  Each saved
@@ -510,20 +527,21 @@ struct BlockFFT2 {
     constexpr int comps = SPLIT ? 2 : 1;
     // opaque copy of the thread index: LDS addresses are recomputed here instead of
     // being hoisted (they are loop-invariant across transforms) and held live
-    int t = t_in;
+    unsigned t = t_in;
     asm volatile("" : "+v"(t));
 #pragma unroll
     for (int c = 0; c < comps; ++c) {
       __syncthreads();
 #pragma unroll
       for (int b = 0; b < E / R; ++b) {
-        int s, j;
+        unsigned s, j;
         bmap<MAPW, TS, NT, L, R, G>(t, b, s, j);
-        const int k = j % Ns;
-        const int base = (j - k) * R + k;
+        const unsigned k = j % Ns;
+        const unsigned base = (j - k) * R + k;
+        const unsigned a0 = LY::at(s, base);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const int idx = LY::at(s, base + r * Ns);
+          const unsigned idx = LY::linear(Ns) ? a0 + r * LY::delta(Ns) : LY::at(s, base + r * Ns);
           if (SPLIT) {
             lds[idx] = c == 0 ? v[b * R + r].x : v[b * R + r].y;
           } else {
@@ -535,11 +553,12 @@ struct BlockFFT2 {
       __syncthreads();
 #pragma unroll
       for (int b = 0; b < E / Rn; ++b) {
-        int s, j;
+        unsigned s, j;
         bmap<MAPR, TS, NT, L, Rn, G>(t, b, s, j);
+        const unsigned a0 = LY::at(s, j);
 #pragma unroll
         for (int r = 0; r < Rn; ++r) {
-          const int idx = LY::at(s, j + r * (L / Rn));
+          const unsigned idx = LY::linear(L / Rn) ? a0 + r * LY::delta(L / Rn) : LY::at(s, j + r * (L / Rn));
           if (SPLIT) {
             if (c == 0)
               v[b * Rn + r].x = lds[idx];
@@ -587,11 +606,11 @@ struct BlockFFT2 {
   // twiddles), one butterfly group at a time, so only one group's loads are live
   template <int R0, class Ld>
   __device__ __forceinline__ static void load_dft(cx<T>* v, int t_in, Ld ld) {
-    int t = t_in;
+    unsigned t = t_in;
     asm volatile("" : "+v"(t));
 #pragma unroll
     for (int b = 0; b < E / R0; ++b) {
-      int s, j;
+      unsigned s, j;
       bmap<MAPF, TS, NT, L, R0, G>(t, b, s, j);
 #pragma unroll
       for (int r = 0; r < R0; ++r) v[b * R0 + r] = ld(s, j + r * (L / R0));
@@ -605,7 +624,7 @@ struct BlockFFT2 {
   __device__ __forceinline__ static void load(cx<T>* v, int t, Ld ld) {
 #pragma unroll
     for (int b = 0; b < E / R0; ++b) {
-      int s, j;
+      unsigned s, j;
       bmap<MAPF, TS, NT, L, R0, G>(t, b, s, j);
 #pragma unroll
       for (int r = 0; r < R0; ++r) v[b * R0 + r] = ld(s, j + r * (L / R0));
@@ -616,7 +635,7 @@ struct BlockFFT2 {
   __device__ __forceinline__ static void store(const cx<T>* v, int t, St st) {
 #pragma unroll
     for (int b = 0; b < E / RL; ++b) {
-      int s, j;
+      unsigned s, j;
       bmap<MAPL, TS, NT, L, RL, G>(t, b, s, j);
 #pragma unroll
       for (int r = 0; r < RL; ++r) st(s, j + r * (L / RL), v[b * RL + r]);

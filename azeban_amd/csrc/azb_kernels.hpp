@@ -1053,37 +1053,44 @@ __global__ __launch_bounds__(P * L / 4, (sizeof(T) == 8 ? 3 : 4)) void euler_row
   const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
   const T nrm = T(a.norm);
 
-  // packed inverse input: sequence s of pair s/3, type s%3
-  auto ld_inv = [&](int s, int pos) -> cx<T> {
-    const int pair = s / 3, type = s % 3;
-    const long la = line0 + 2 * pair, lb = la + 1;
-    int kk;
-    int mode;  // 1: direct, 2: mirrored, 3: k = 0
-    if (pos == 0) {
-      kk = 0; mode = 3;
-    } else if (pos <= N / 2) {
-      kk = pos; mode = 1;
-    } else if (pos >= L - N / 2) {
-      kk = L - pos; mode = 2;
-    } else {
-      return mk(T(0), T(0));
-    }
-    long ix, iy;
-    if (type == 0) {
-      ix = (0 * a.nlines + la) * a.ls; iy = (1 * a.nlines + la) * a.ls;
-    } else if (type == 1) {
-      ix = (2 * a.nlines + la) * a.ls; iy = (2 * a.nlines + lb) * a.ls;
-    } else {
-      ix = (0 * a.nlines + lb) * a.ls; iy = (1 * a.nlines + lb) * a.ls;
-    }
-    const cx<T> X0 = in[ix + kk], Y0 = in[iy + kk];
-    if (mode == 1) return mk(X0.x - Y0.y, X0.y + Y0.x);
-    if (mode == 2) return mk(X0.x + Y0.y, Y0.x - X0.y);
-    return mk(X0.x, Y0.x);
-  };
-
+  // First inverse pass, loaded straight from HBM: radix 12 with one butterfly per
+  // thread (MAP_SEQ1), butterfly j of sequence s at positions j + NB r.  Sequence s
+  // is type s % 3 of pair s / 3: (u1a + i u2a, u3a + i u3b, u1b + i u2b), packed from
+  // the half spectra X, Y of its two real lines.  Positions up to N/2 = 4 NB read X, Y
+  // at k = pos (r < 4; r = 4 only for j = 0), positions from L - N/2 = 8 NB read the
+  // conjugates at k = L - pos (r >= 8), the rest are the zero padding.
   cx<T> v[E];
-  FI::template load_dft<12>(v, t, ld_inv);
+  {
+    unsigned tt = t;
+    asm volatile("" : "+v"(tt));
+    const unsigned s = tt / NB, j = tt % NB;
+    const unsigned pair = s / 3, type = s % 3;
+    const long la = line0 + 2 * pair, lb = la + 1;
+    const long lx = type == 0 ? la : (type == 1 ? 2 * a.nlines + la : lb);
+    const long ly = type == 0 ? a.nlines + la : (type == 1 ? 2 * a.nlines + lb : a.nlines + lb);
+    const cx<T>* __restrict__ px = in + lx * a.ls;
+    const cx<T>* __restrict__ py = in + ly * a.ls;
+    const bool j0 = j == 0;
+#pragma unroll
+    for (int r = 0; r < 12; ++r) {
+      if (r < 4) {
+        const cx<T> X0 = px[j + NB * r], Y0 = py[j + NB * r];
+        v[r] = mk(X0.x - Y0.y, X0.y + Y0.x);
+        if (r == 0 && j0) v[r] = mk(X0.x, Y0.x);  // k = 0: c2r ignores Im
+      } else if (r >= 8) {
+        const unsigned kk = NB * (12 - r) - j;
+        const cx<T> X0 = px[kk], Y0 = py[kk];
+        v[r] = mk(X0.x + Y0.y, Y0.x - X0.y);
+      } else {
+        v[r] = mk(T(0), T(0));
+        if (r == 4 && j0) {
+          const cx<T> X0 = px[4 * NB], Y0 = py[4 * NB];
+          v[r] = mk(X0.x - Y0.y, X0.y + Y0.x);
+        }
+      }
+    }
+    DFT<12, +1, T>::run(v);
+  }
   FI::run_rest(v, t, X, tw, typename PL::Inv{});
   // u[g*4 + r]: packed sequence type g of this thread's pair at its 4 positions
   cx<T> u[E];
