@@ -25,6 +25,30 @@ struct cx {
   T x, y;
 };
 
+// Complex elements as a native two-vector, for 8/16-byte loads and stores through
+// pointers in the global address space: a uniform (scalar) base plus a 32-bit
+// element offset, with no 64-bit address arithmetic per element.
+template <typename T>
+using vec2_t = T __attribute__((ext_vector_type(2)));
+template <typename T>
+using gvec2 = __attribute__((address_space(1))) vec2_t<T>;
+
+template <typename T>
+__device__ __forceinline__ cx<T> gload(const void* base, unsigned idx) {
+  const vec2_t<T> v = ((const gvec2<T>*)base)[idx];
+  cx<T> r;
+  r.x = v.x;
+  r.y = v.y;
+  return r;
+}
+template <typename T>
+__device__ __forceinline__ void gstore(void* base, unsigned idx, T x, T y) {
+  vec2_t<T> v;
+  v.x = x;
+  v.y = y;
+  ((gvec2<T>*)base)[idx] = v;
+}
+
 template <typename T>
 __device__ __forceinline__ cx<T> mk(T a, T b) {
   cx<T> r;

@@ -98,7 +98,7 @@ constexpr int col_ts() {
 }
 
 template <typename Tin, typename Tout, typename T, int L, int TS, int DIR, int MODE>
-__global__ __launch_bounds__(TS * L / LenPlan<L>::E, (sizeof(T) == 4 ? 4 : (MODE == COL_PAD_IN && (L == 768 || L == 384) ? 3 : 2))) void col_fft_kernel(ColArgs a) {
+__global__ __launch_bounds__(TS * L / LenPlan<L>::E, (sizeof(T) == 4 ? 4 : (L == 768 || L == 384 ? 3 : 2))) void col_fft_kernel(ColArgs a) {
   using LP = LenPlan<L>;
   constexpr int E = LP::E;
   constexpr int NT = TS * L / E;
@@ -115,49 +115,48 @@ __global__ __launch_bounds__(TS * L / LenPlan<L>::E, (sizeof(T) == 4 ? 4 : (MODE
   const long batch = rest / a.nq;
   if (a.st[batch / a.bps].done) return;
   const long c0 = (long)kt * TS;
-  const int N = a.nsmall;
-  const int half = N / 2;
-  const cx<Tin>* __restrict__ in = (const cx<Tin>*)a.in + batch * a.in_bs + q * a.in_qs + c0;
-  cx<Tout>* __restrict__ out = (cx<Tout>*)a.out + batch * a.out_bs + q * a.out_qs + c0;
-  const long ncols_left = a.nk - c0;
+  // element offsets from the tile's base stay below 2^32 (checked at launch)
+  const void* in = (const cx<Tin>*)a.in + batch * a.in_bs + q * a.in_qs + c0;
+  void* out = (cx<Tout>*)a.out + batch * a.out_bs + q * a.out_qs + c0;
+  const unsigned irs = (unsigned)a.in_rs, ors = (unsigned)a.out_rs;
+  const unsigned ncols_left = (unsigned)(a.nk - c0);
+  // pruned length of the padded/truncated side (M = 3N/2, checked at launch)
+  constexpr unsigned N = MODE == COL_FULL ? L : 2 * L / 3, half = N / 2;
   const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
 
   cx<T> v[E];
   // all first-pass loads are issued before any use (maximum bytes in flight)
-  F::template load<R0>(v, t, [&](int s, int pos) -> cx<T> {
-    long row = pos;
+  F::template load<R0>(v, t, [&](unsigned s, unsigned pos) -> cx<T> {
+    unsigned row = pos;
+    bool valid = s < ncols_left;
     if (MODE == COL_PAD_IN) {
-      if (pos <= half)
-        row = pos;
-      else if (pos >= L - half + 1)
-        row = pos - (L - N);
-      else
-        row = -1;
+      if (pos > half) {
+        if (pos >= L - half + 1)
+          row = pos - (L - N);
+        else
+          valid = false;
+      }
     }
     cx<T> r = mk(T(0), T(0));
-    if (row >= 0 && s < ncols_left) {
-      cx<Tin> x = in[row * a.in_rs + s];
+    if (valid) {
+      const cx<Tin> x = gload<Tin>(in, row * irs + s);
       r = mk(T(x.x), T(x.y));
     }
     return r;
   });
   F::run(v, t, lds, tw, typename LP::P{});
-  F::template store<RL>(v, t, [&](int s, int pos, cx<T> val) {
-    long row = pos;
+  F::template store<RL>(v, t, [&](unsigned s, unsigned pos, cx<T> val) {
+    unsigned row = pos;
+    bool valid = s < ncols_left;
     if (MODE == COL_TRUNC_OUT) {
-      if (pos <= half)
-        row = pos;
-      else if (pos >= L - half + 1)
-        row = pos - (L - N);
-      else
-        row = -1;
+      if (pos > half) {
+        if (pos >= L - half + 1)
+          row = pos - (L - N);
+        else
+          valid = false;
+      }
     }
-    if (row >= 0 && s < ncols_left) {
-      cx<Tout> o;
-      o.x = Tout(val.x);
-      o.y = Tout(val.y);
-      out[row * a.out_rs + s] = o;
-    }
+    if (valid) gstore<Tout>(out, row * ors + s, Tout(val.x), Tout(val.y));
   });
 }
 
@@ -810,185 +809,6 @@ __global__ void energy_spectrum_kernel(const double2* u, int dim, int N, long nm
 namespace azb {
 
 // ---------------------------------------------------------------------------
-// 3D row kernel, register-resident products.  Per workgroup: P line pairs
-// (a, b), TS = 3P packed sequences, NT = P*L/8 threads, 24 complex per thread.
-// Each of two rounds runs
-//   inverse  : packed c2r of (u1a + i u2a, u3a + i u3b, u1b + i u2b), first pass
-//              loaded straight from HBM, last pass MAP_SEQ so every thread holds
-//              u1, u2, u3 of both lines at the same 8 positions;
-//   products : round 0 -> (B11, B21, B22), round 1 -> (B31, B32, B33), packed
-//              (line a) + i (line b), in registers, feeding the forward FFT's
-//              MAP_SEQ first pass at exactly those positions;
-//   forward  : last pass MAP_PAIR, so Z[k] and Z[L-k] are in one thread: the two
-//              half spectra are separated in registers and k <= N/2 is stored.
-// The inverse is recomputed in round 1 instead of keeping u in LDS, which leaves
-// LDS to the FFT exchange alone (TS*L words).
-// ---------------------------------------------------------------------------
-template <int L>
-struct Row3Plan;
-template <> struct Row3Plan<768> { using Inv = Plan<8, 12, 8>; using Fwd = Plan<8, 8, 12>; static constexpr int INV_MAPF = MAP_SEQ; };
-template <> struct Row3Plan<384> { using Inv = Plan<8, 6, 8>; using Fwd = Plan<8, 4, 12>; static constexpr int INV_MAPF = MAP_SEQ; };
-template <> struct Row3Plan<192> { using Inv = Plan<8, 3, 8>; using Fwd = Plan<8, 2, 12>; static constexpr int INV_MAPF = MAP_SEQ; };
-template <> struct Row3Plan<96> { using Inv = Plan<12, 8>; using Fwd = Plan<8, 12>; static constexpr int INV_MAPF = MAP_STD; };
-
-template <typename T, int L, int P>
-struct Row3Cfg {
-  static constexpr int TS = 3 * P;
-  static constexpr int NT = P * L / 8;
-  static constexpr int E = 24;
-  static constexpr int N = 2 * L / 3;
-  static constexpr int NH = N / 2 + 1;
-  // FP64 is register-bound to two workgroups per CU, whose interleaved exchange
-  // buffers (2 x 74 KB at L = 768) still fit in LDS: one barrier pair per exchange
-  static constexpr bool SPLIT = false;
-  static constexpr int XWORDS = 2 * ExLayout<T, L, TS, true>::elems;
-};
-
-template <typename T, int L, int P>
-__global__ __launch_bounds__(P * L / 8, (sizeof(T) == 8 ? 2 : 4)) void euler_row3_kernel(RowArgs a) {
-  using C = Row3Cfg<T, L, P>;
-  using PL = Row3Plan<L>;
-  constexpr int NT = C::NT, E = C::E, N = C::N, NH = C::NH, TS = C::TS;
-  using FI = BlockFFT2<T, L, TS, NT, +1, C::SPLIT, 3, PL::INV_MAPF, MAP_SEQ, true>;
-  using FF = BlockFFT2<T, L, TS, NT, -1, C::SPLIT, 3, MAP_SEQ, MAP_PAIR, true>;
-  static_assert(FI::lds_words == C::XWORDS && FF::lds_words == C::XWORDS, "exchange size");
-  constexpr int RI0 = first_radix(typename PL::Inv{});
-  static_assert(last_radix(typename PL::Inv{}) == 8 && first_radix(typename PL::Fwd{}) == 8,
-                "products need the inverse's last and the forward's first pass on radix 8");
-  constexpr int RL = last_radix(typename PL::Fwd{});
-  __shared__ T X[C::XWORDS];
-  __shared__ double red[NT / 64 + 1];
-  const long bps = a.nlines / (2 * P);  // workgroups per sample
-  const long sample = blockIdx.x / bps;
-  DevState* st = a.st + sample;
-  if (st->done) return;
-
-  const int t = threadIdx.x;
-  const long line0 = (blockIdx.x % bps) * (2 * P);
-  const cx<T>* __restrict__ in = (const cx<T>*)a.in + sample * a.in_ss;
-  cx<T>* __restrict__ out = (cx<T>*)a.out + sample * a.out_ss;
-  const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
-  const T nrm = T(a.norm);
-  double lmax = 0.0;
-
-  // packed inverse input: sequence s of pair s/3, type s%3
-  auto ld_inv = [&](int s, int pos) -> cx<T> {
-    const int pair = s / 3, type = s % 3;
-    const long la = line0 + 2 * pair, lb = la + 1;
-    int kk;
-    int mode;  // 0: zero, 1: direct, 2: mirrored, 3: k = 0
-    if (pos == 0) {
-      kk = 0; mode = 3;
-    } else if (pos <= N / 2) {
-      kk = pos; mode = 1;
-    } else if (pos >= L - N / 2) {
-      kk = L - pos; mode = 2;
-    } else {
-      return mk(T(0), T(0));
-    }
-    long ix, iy;
-    if (type == 0) {
-      ix = (0 * a.nlines + la) * a.ls; iy = (1 * a.nlines + la) * a.ls;
-    } else if (type == 1) {
-      ix = (2 * a.nlines + la) * a.ls; iy = (2 * a.nlines + lb) * a.ls;
-    } else {
-      ix = (0 * a.nlines + lb) * a.ls; iy = (1 * a.nlines + lb) * a.ls;
-    }
-    const cx<T> X0 = in[ix + kk], Y0 = in[iy + kk];
-    if (mode == 1) return mk(X0.x - Y0.y, X0.y + Y0.x);
-    if (mode == 2) return mk(X0.x + Y0.y, Y0.x - X0.y);
-    return mk(X0.x, Y0.x);
-  };
-
-  // FP32 keeps the inverse output in registers for the second product round
-  // (48 more VGPRs); FP64 would exceed the register file and recomputes it
-  constexpr bool KEEP_U = sizeof(T) == 4;
-  cx<T> v[E];
-  cx<T> ukeep[KEEP_U ? E : 1];
-#pragma unroll 1
-  for (int round = 0; round < 2; ++round) {
-    if (!KEEP_U || round == 0) {
-      FI::template load_dft<RI0>(v, t, ld_inv);
-      FI::run_rest(v, t, X, tw, typename PL::Inv{});
-      if (KEEP_U) {
-#pragma unroll
-        for (int e = 0; e < E; ++e) ukeep[KEEP_U ? e : 0] = v[e];
-      }
-    } else {
-#pragma unroll
-      for (int e = 0; e < E; ++e) v[e] = ukeep[KEEP_U ? e : 0];
-    }
-    // v[g*8 + r]: packed sequence type g of this thread's pair, same 8 positions
-#pragma unroll
-    for (int r = 0; r < 8; ++r) {
-      const cx<T> p0 = v[r], p1 = v[8 + r], p2 = v[16 + r];
-      // p0 = (u1a, u2a), p1 = (u3a, u3b), p2 = (u1b, u2b)
-      if (round == 0) {
-        lmax = fmax(lmax, (double)fmax(fmax(fabs(p0.x), fabs(p0.y)), fmax(fabs(p2.x), fabs(p2.y))));
-        const T a1 = nrm * p0.x, a2 = nrm * p0.y, b1 = nrm * p2.x, b2 = nrm * p2.y;
-        v[r] = mk(a1 * a1, b1 * b1);       // B11
-        v[8 + r] = mk(a2 * a1, b2 * b1);   // B21
-        v[16 + r] = mk(a2 * a2, b2 * b2);  // B22
-      } else {
-        lmax = fmax(lmax, (double)fmax(fabs(p1.x), fabs(p1.y)));
-        const T a1 = nrm * p0.x, a2 = nrm * p0.y, b1 = nrm * p2.x, b2 = nrm * p2.y;
-        const T a3 = nrm * p1.x, b3 = nrm * p1.y;
-        v[r] = mk(a3 * a1, b3 * b1);       // B31
-        v[8 + r] = mk(a3 * a2, b3 * b2);   // B32
-        v[16 + r] = mk(a3 * a3, b3 * b3);  // B33
-      }
-    }
-    FF::run(v, t, X, tw, typename PL::Fwd{});
-    // separation: one MAP_PAIR unit per thread: butterflies j0 = p, j1 = mirror
-    {
-      constexpr int H = L / (2 * RL);
-      int tt = t;
-      asm volatile("" : "+v"(tt));
-      const int s = tt / H, p = tt % H;
-      const int pair = s / 3, m = round * 3 + s % 3;
-      const long la = line0 + 2 * pair, lb = la + 1;
-      cx<T>* oa = out + ((long)m * a.nlines + la) * a.ls;
-      cx<T>* ob = out + ((long)m * a.nlines + lb) * a.ls;
-      // register indices must be compile-time: the p == 0 unit (butterflies 0 and
-      // H, each its own mirror) and the general unit are unrolled separately
-      // streaming stores: keep L2 for the inputs, which round 1 reads again
-      auto emit = [&](int pos, const cx<T>& zk, const cx<T>& zm) {
-        if (pos <= N / 2) {
-          __builtin_nontemporal_store(T(0.5) * (zk.x + zm.x), &oa[pos].x);
-          __builtin_nontemporal_store(T(0.5) * (zk.y - zm.y), &oa[pos].y);
-          __builtin_nontemporal_store(T(0.5) * (zk.y + zm.y), &ob[pos].x);
-          __builtin_nontemporal_store(T(-0.5) * (zk.x - zm.x), &ob[pos].y);
-        }
-      };
-      if (p != 0) {
-#pragma unroll
-        for (int r = 0; r < RL; ++r) {
-          emit(p + (L / RL) * r, v[r], v[RL + RL - 1 - r]);
-          emit(L / RL - p + (L / RL) * r, v[RL + r], v[RL - 1 - r]);
-        }
-      } else {
-#pragma unroll
-        for (int r = 0; r < RL; ++r) {
-          emit((L / RL) * r, v[r], v[(RL - r) % RL]);
-          emit(H + (L / RL) * r, v[RL + r], v[RL + RL - 1 - r]);
-        }
-      }
-    }
-  }
-
-  lmax = wave_max<T>(lmax);
-  if ((t & 63) == 0) red[t >> 6] = lmax;
-  __syncthreads();
-  if (t == 0) {
-    double mx = 0.0;
-    for (int w = 0; w < (NT + 63) / 64; ++w) mx = fmax(mx, red[w]);
-    atomicMax(&st->umax_bins[a.slot][blockIdx.x & (kBins - 1)],
-              (unsigned long long)__double_as_longlong(mx));
-  }
-}
-
-
-// ---------------------------------------------------------------------------
 // 3D row kernel, 12 values per thread.  Per workgroup: P line pairs (a, b),
 // TS = 3P packed sequences, NT = P*L/4 threads.
 //   inverse  : packed c2r of (u1a + i u2a, u3a + i u3b, u1b + i u2b), run once.  Its
@@ -1024,11 +844,8 @@ struct Row4Cfg {
   static constexpr int XWORDS = 2 * ExLayout<T, L, TS, true, SOFF>::elems;
 };
 
-template <typename T>
-using vec2_t = T __attribute__((ext_vector_type(2)));
-
 template <typename T, int L, int P>
-__global__ __launch_bounds__(P * L / 4, (sizeof(T) == 8 ? 3 : 4)) void euler_row4_kernel(RowArgs a) {
+__global__ __launch_bounds__(P * L / 4) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 3 : 4, sizeof(T) == 8 ? 3 : 4))) void euler_row4_kernel(RowArgs a) {
   using C = Row4Cfg<T, L, P>;
   using PL = Row4Plan<L>;
   constexpr int NT = C::NT, E = C::E, N = C::N, TS = C::TS, NB = C::NB;
@@ -1100,6 +917,9 @@ __global__ __launch_bounds__(P * L / 4, (sizeof(T) == 8 ? 3 : 4)) void euler_row
     u[e] = v[e];
     lmax = fmax(lmax, (double)fmax(fabs(v[e].x), fabs(v[e].y)));
   }
+  // settle the max here: left to itself the compiler sinks the reduction past the
+  // product rounds and keeps its inputs alive (spilled) across them
+  asm volatile("" : "+v"(lmax));
 
 #pragma unroll 1
   for (int round = 0; round < 2; ++round) {

@@ -3,6 +3,7 @@
 // loop, and the standalone FFT, pad, axpby and max-norm seams.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -92,6 +93,12 @@ void launch_col_L(ColArgs a, long batches, hipStream_t st, int tsdiv = 1) {
 
 template <typename Tin, typename Tout, typename T, int DIR, int MODE>
 int launch_col(int L, ColArgs a, long batches, hipStream_t st, int tsdiv = 1) {
+  // the kernels take the pruned length as 2L/3 and address a tile with 32-bit
+  // element offsets from its base
+  if (MODE != COL_FULL && 3L * a.nsmall != 2L * L)
+    return fail(AZB_ERR_UNSUPPORTED, "pruned column passes need M = 3N/2");
+  if ((double)(L - 1) * (double)std::max(a.in_rs, a.out_rs) + (double)L >= 4294967296.0)
+    return fail(AZB_ERR_UNSUPPORTED, "column strides exceed 32-bit element offsets");
   switch (L) {
 #define AZB_CASE(LL) \
   case LL: launch_col_L<Tin, Tout, T, LL, DIR, MODE>(a, batches, st, tsdiv); break;
@@ -127,29 +134,18 @@ void launch_row_P(RowArgs a, long nsamples, hipStream_t st) {
 }
 
 template <typename T, int L, int P>
-void launch_row3_P(RowArgs a, long nsamples, hipStream_t st) {
-  const long blocks = a.nlines / (2 * P) * nsamples;
-  hipLaunchKernelGGL((euler_row3_kernel<T, L, P>), dim3((unsigned)blocks), dim3(P * L / 8), 0, st, a);
-}
-
-template <typename T, int L, int P>
 void launch_row4_P(RowArgs a, long nsamples, hipStream_t st) {
   const long blocks = a.nlines / (2 * P) * nsamples;
   hipLaunchKernelGGL((euler_row4_kernel<T, L, P>), dim3((unsigned)blocks), dim3(Row4Cfg<T, L, P>::NT), 0, st, a);
 }
 
 // pdiv in {1, 2}: P0 = 1536/L line pairs per workgroup, or half of that
-// (3D: 3 and 4 select the 24-values-per-thread kernel with P0 / P1)
 template <typename T, int L, int DIM>
 void launch_row_L(RowArgs a, long nsamples, hipStream_t st, int pdiv) {
   constexpr int P0 = (1536 / L) < 1 ? 1 : 1536 / L;
   constexpr int P1 = P0 / 2 >= 1 ? P0 / 2 : 1;
   if constexpr (DIM == 3) {
-    if (pdiv == 4)
-      launch_row3_P<T, L, P1>(a, nsamples, st);
-    else if (pdiv == 3)
-      launch_row3_P<T, L, P0>(a, nsamples, st);
-    else if (pdiv == 2)
+    if (pdiv == 2)
       launch_row4_P<T, L, P1>(a, nsamples, st);
     else
       launch_row4_P<T, L, P0>(a, nsamples, st);
@@ -164,7 +160,7 @@ void launch_row_L(RowArgs a, long nsamples, hipStream_t st, int pdiv) {
 template <typename T, int DIM>
 int launch_row(int L, RowArgs a, long nsamples, hipStream_t st, int pdiv = 1) {
   int P = row_p(L);
-  if ((pdiv == 2 || pdiv == 4) && P >= 2) P /= 2;
+  if (pdiv == 2 && P >= 2) P /= 2;
   if (a.nlines % (2 * P) != 0) return fail(AZB_ERR_UNSUPPORTED, "line count not a multiple of the line-pair tile");
   if (DIM == 3 && L == 1536) return fail(AZB_ERR_UNSUPPORTED, "3D row length 1536 not built");
   switch (L) {
@@ -790,11 +786,10 @@ int azb_set_tuning(azb_solver* s, const int* tsdiv, int pdiv) {
     if (v != 1 && v != 2 && v != 4) return fail(AZB_ERR_ARG, "tile divisors must be 1, 2 or 4");
     s->tsdiv[i] = v;
   }
-  if (pdiv < 1 || pdiv > (s->dim == 3 ? 4 : 2)) return fail(AZB_ERR_ARG, "row variant out of range");
+  if (pdiv != 1 && pdiv != 2) return fail(AZB_ERR_ARG, "row divisor must be 1 or 2");
   const int P = row_p(s->M);
   const long lines = s->dim == 3 ? (long)s->M * s->M : (long)s->M;
-  const bool half = pdiv == 2 || pdiv == 4;
-  s->pdiv = (half && !(P >= 2 && lines % P == 0)) ? pdiv - 1 : pdiv;
+  s->pdiv = (pdiv == 2 && P >= 2 && lines % P == 0) ? 2 : 1;
   return AZB_OK;
 }
 

@@ -47,6 +47,46 @@ def dsl_device(N: int, seed: int = 1):
     return torch.from_numpy(I.double_shear_layer(N, seed=seed)).to("cuda")
 
 
+# kernel of each family at the 512^3 FP64 workload, as named in the counter summary
+_FAMILY_KERNEL = {"row_fused": "azb::euler_row4_kernel<double, 768,", "kspace_rk": "azb::kspace_kernel<double, 3>"}
+COUNTERS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "counters_tg3d_512.json")
+
+
+def counted_traffic(dim: int, N: int, family: str):
+    """HBM bytes per launch of the family's kernel from the committed counter summary
+    (FETCH_SIZE and WRITE_SIZE passes of rocprofv3, scaled by a copy of known size);
+    None when no summary for this workload and kernel exists."""
+    if (dim, N) != (3, 512) or family not in _FAMILY_KERNEL or not os.path.exists(COUNTERS_FILE):
+        return None
+    with open(COUNTERS_FILE) as f:
+        summ = json.load(f)
+    for name, rec in summ["kernels"].items():
+        if name.startswith(_FAMILY_KERNEL[family]):
+            return round(rec["hbm_bytes"], 0)
+    return None
+
+
+def measured_copy_bandwidth(gib: float = 4.0, reps: int = 5) -> float:
+    """Achievable HBM bandwidth: a device-to-device copy (read + write) of `gib` GiB."""
+    import torch
+
+    n = int(gib * 2**30) // 8
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    y.copy_(x)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        y.copy_(x)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del x, y
+    torch.cuda.empty_cache()
+    return 2 * n * 8 / (ms * 1e-3) / 1e9
+
+
 def run_precision(A, dim, N, u_hat0, precision, steps, warmup):
     import torch
 
@@ -276,6 +316,7 @@ def main():
         return ms_step, kern
 
     ms64, kern64 = summarize(res["fp64"], "fp64")
+    copy_gbs = measured_copy_bandwidth()
     value = pts * args.steps / (res["fp64"]["gpu_ms"] * 1e-3) / 1e6
     dom = max(kern64.items(), key=lambda kv: kv[1]["ms_avg"] * kv[1]["launches"])
     step_gb = traffic.step_bytes(dim, N, "fp64") / 1e9
@@ -286,7 +327,9 @@ def main():
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": dom[1]["frac_hbm"],
-        "traffic": None,
+        "traffic": counted_traffic(dim, N, dom[0]),
+        "measured_copy_GB_s": round(copy_gbs, 1),
+        "frac_of_measured_copy": round(dom[1]["GB_s"] / copy_gbs, 4),
         "step_algorithmic_GB": round(step_gb, 3),
         "step_GB_s": round(step_gb / (ms64 * 1e-3), 1),
     }
