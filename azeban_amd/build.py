@@ -23,6 +23,11 @@ HIPCC_FLAGS = [
     "-std=c++17",
     "-fPIC",
     "-shared",
+    # multiply-adds are fused only within one source expression: the backend's
+    # cross-statement fusion depends on which outputs a kernel instantiation
+    # discards (e.g. the truncated rows of a forward column pass for a given tile
+    # width), which would make results differ in the last bit between launch shapes
+    "-ffp-contract=on",
     # no packed-FP32 arithmetic in the generated gfx950 code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]

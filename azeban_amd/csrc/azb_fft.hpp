@@ -68,6 +68,13 @@ template <typename T>
 __device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) {
   return mk(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
+// a b with the two fused multiply-adds fixed in the source, so that every
+// instantiation (any tile width or thread mapping) rounds identically
+template <typename T>
+__device__ __forceinline__ cx<T> cmul_fixed(cx<T> a, cx<T> b) {
+  const T yy = a.y * b.y, yx = a.y * b.x;
+  return mk(T(fma(a.x, b.x, -yy)), T(fma(a.x, b.y, yx)));
+}
 template <typename T>
 __device__ __forceinline__ cx<T> cscale(cx<T> a, T s) {
   return mk(a.x * s, a.y * s);
@@ -530,11 +537,11 @@ struct BlockFFT2 {
           // chain costs four FMA-class operations and keeps two powers live.
           const cx<T> w1 = twiddle(tw, k * step);
           cx<T> wr = w1;
-          w[1] = cmul(w[1], w1);
+          w[1] = cmul_fixed(w[1], w1);
 #pragma unroll
           for (int r = 2; r < R; ++r) {
-            wr = cmul(wr, w1);
-            w[r] = cmul(w[r], wr);
+            wr = cmul_fixed(wr, w1);
+            w[r] = cmul_fixed(w[r], wr);
           }
         } else {
           // FP32 loads every power: a chain's error would rival the transform's own

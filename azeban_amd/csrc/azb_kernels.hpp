@@ -97,6 +97,17 @@ constexpr int col_ts() {
                       : ((4096 * (8 / (int)sizeof(T))) / L < 1 ? 1 : (4096 * (8 / (int)sizeof(T))) / L);
 }
 
+// Workgroups are handed to the 8 XCDs round-robin (workgroup b to XCD b mod 8),
+// and each XCD has its own L2.  Renumbering so that each XCD takes one contiguous
+// range of logical workgroups puts neighbouring column tiles (which share the
+// 128-byte lines that straddle their boundary when rows are not line-aligned, as
+// in the caller's N/2+1-wide state) in the same L2.  A bijection for any count.
+__device__ __forceinline__ unsigned xcd_contiguous(unsigned b, unsigned nb) {
+  constexpr unsigned X = 8;
+  const unsigned x = b % X, i = b / X, q = nb / X, r = nb % X;
+  return x * q + (x < r ? x : r) + i;
+}
+
 template <typename Tin, typename Tout, typename T, int L, int TS, int DIR, int MODE>
 __global__ __launch_bounds__(TS * L / LenPlan<L>::E, (sizeof(T) == 4 ? 4 : (L == 768 || L == 384 ? 3 : 2))) void col_fft_kernel(ColArgs a) {
   using LP = LenPlan<L>;
@@ -109,8 +120,9 @@ __global__ __launch_bounds__(TS * L / LenPlan<L>::E, (sizeof(T) == 4 ? 4 : (L ==
   constexpr int RL = last_radix(typename LP::P{});
   __shared__ T lds[ExchangeSize<T, L, TS, SPLIT>::words];
   const int t = threadIdx.x;
-  const int kt = blockIdx.x % a.ktiles;
-  const long rest = blockIdx.x / a.ktiles;
+  const unsigned blk = xcd_contiguous(blockIdx.x, gridDim.x);
+  const int kt = blk % a.ktiles;
+  const long rest = blk / a.ktiles;
   const long q = rest % a.nq;
   const long batch = rest / a.nq;
   if (a.st[batch / a.bps].done) return;

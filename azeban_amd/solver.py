@@ -96,7 +96,10 @@ class IncompressibleEuler:
     def __del__(self):
         h = getattr(self, "handle", None)
         if h:
-            load().azb_destroy(h)
+            try:
+                load().azb_destroy(h)
+            except Exception:  # interpreter teardown: the process releases the device anyway
+                pass
             self.handle = None
 
     def n_vars(self) -> int:
