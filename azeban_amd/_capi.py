@@ -66,6 +66,7 @@ SIGNATURES = {
     "azb_copy_to_padded": (_i, [_i, _i, _i, _vp, _vp, _vp]),
     "azb_copy_from_padded": (_i, [_i, _i, _i, _vp, _vp, _vp]),
     "azb_scale": (_i, [_d, _vp, _l, _vp]),
+    "azb_copy_bandwidth": (_i, [_d, _i, C.POINTER(_d)]),
     "azb_axpby": (_i, [_d, _vp, _d, _vp, _l, _vp]),
     "azb_max_norm": (_i, [_vp, _l, _dp, _vp]),
     "azb_leray": (_i, [_i, _i, _vp, _vp]),

@@ -244,14 +244,50 @@ struct DFTComposite {
   }
 };
 
+// Prime-factor (Good-Thomas) radix R = P*Q with gcd(P, Q) = 1: input index
+// n = (Q n1 + P n2) mod R, output index k = (A k1 + B k2) mod R with A = 1 (mod P),
+// 0 (mod Q) and B = 0 (mod P), 1 (mod Q).  Then n k = Q n1 k1 + P n2 k2 (mod R), so
+//   X[A k1 + B k2] = sum_n2 W_Q^{n2 k2} sum_n1 W_P^{n1 k1} x[Q n1 + P n2]
+// needs no twiddles between the two DFT stages (the index maps are register renames).
+constexpr int ce_inverse_mod(int a, int m) {
+  for (int x = 1; x <= m; ++x)
+    if ((a * x) % m == 1) return x;
+  return 0;
+}
+template <int P, int Q, int DIR, typename T>
+struct DFTPrimeFactor {
+  static constexpr int R = P * Q;
+  static constexpr int A = (Q * ce_inverse_mod(Q % P, P)) % R;
+  static constexpr int B = (P * ce_inverse_mod(P % Q, Q)) % R;
+  static_assert(A % P == 1 % P && A % Q == 0 && B % P == 0 && B % Q == 1 % Q, "P and Q must be coprime");
+  __device__ __forceinline__ static void run(cx<T>* v) {
+    cx<T> y[Q][P];
+#pragma unroll
+    for (int n2 = 0; n2 < Q; ++n2) {
+#pragma unroll
+      for (int n1 = 0; n1 < P; ++n1) y[n2][n1] = v[(Q * n1 + P * n2) % R];
+      DFT<P, DIR, T>::run(y[n2]);
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < P; ++k1) {
+      cx<T> z[Q];
+#pragma unroll
+      for (int n2 = 0; n2 < Q; ++n2) z[n2] = y[n2][k1];
+      DFT<Q, DIR, T>::run(z);
+#pragma unroll
+      for (int k2 = 0; k2 < Q; ++k2) v[(A * k1 + B * k2) % R] = z[k2];
+    }
+  }
+};
+
 template <int DIR, typename T>
-struct DFT<6, DIR, T> : DFTComposite<2, 3, DIR, T> {};
+struct DFT<6, DIR, T> : DFTPrimeFactor<2, 3, DIR, T> {};
 template <int DIR, typename T>
-struct DFT<12, DIR, T> : DFTComposite<4, 3, DIR, T> {};
+struct DFT<12, DIR, T> : DFTPrimeFactor<4, 3, DIR, T> {};
 template <int DIR, typename T>
 struct DFT<16, DIR, T> : DFTComposite<4, 4, DIR, T> {};
 template <int DIR, typename T>
-struct DFT<24, DIR, T> : DFTComposite<8, 3, DIR, T> {};
+struct DFT<24, DIR, T> : DFTPrimeFactor<8, 3, DIR, T> {};
 
 // ---------------------------------------------------------------------------
 // Workgroup Stockham FFT
@@ -303,10 +339,10 @@ struct BlockFFT {
     for (int b = 0; b < E / R; ++b) {
       cx<T>* w = v + b * R;
       if (Ns > 1) {
-        const int g = t + NT * b;
-        const int j = g / TS;
-        const int k = j % Ns;
-        constexpr int step = L / (Ns * R);
+        const unsigned g = (unsigned)t + NT * b;
+        const unsigned j = g / TS;
+        const unsigned k = j % Ns;
+        constexpr unsigned step = L / (Ns * R);
 #pragma unroll
         for (int r = 1; r < R; ++r) w[r] = cmul(w[r], twiddle((twptr)tw, r * k * step));
       }
@@ -324,14 +360,14 @@ struct BlockFFT {
       if (active) {
 #pragma unroll
         for (int b = 0; b < E / R; ++b) {
-          const int g = t + NT * b;
-          const int s = g % TS;
-          const int j = g / TS;
-          const int k = j % Ns;
-          const int base = (j - k) * R + k;
+          const unsigned g = (unsigned)t + NT * b;
+          const unsigned s = g % TS;
+          const unsigned j = g / TS;
+          const unsigned k = j % Ns;
+          const unsigned base = (j - k) * R + k;
 #pragma unroll
           for (int r = 0; r < R; ++r) {
-            const int idx = (base + r * Ns) * TS + s;
+            const unsigned idx = (base + r * Ns) * TS + s;
             if (SPLIT) {
               lds[idx] = c == 0 ? v[b * R + r].x : v[b * R + r].y;
             } else {
@@ -345,12 +381,12 @@ struct BlockFFT {
       if (active) {
 #pragma unroll
         for (int b = 0; b < E / Rn; ++b) {
-          const int g = t + NT * b;
-          const int s = g % TS;
-          const int j = g / TS;
+          const unsigned g = (unsigned)t + NT * b;
+          const unsigned s = g % TS;
+          const unsigned j = g / TS;
 #pragma unroll
           for (int r = 0; r < Rn; ++r) {
-            const int idx = (j + r * (L / Rn)) * TS + s;
+            const unsigned idx = (j + r * (L / Rn)) * TS + s;
             if (SPLIT) {
               if (c == 0)
                 v[b * Rn + r].x = lds[idx];
@@ -393,9 +429,9 @@ struct BlockFFT {
   __device__ __forceinline__ static void load(cx<T>* v, int t, Ld ld) {
 #pragma unroll
     for (int b = 0; b < E / R0; ++b) {
-      const int g = t + NT * b;
-      const int s = g % TS;
-      const int j = g / TS;
+      const unsigned g = (unsigned)t + NT * b;
+      const unsigned s = g % TS;
+      const unsigned j = g / TS;
 #pragma unroll
       for (int r = 0; r < R0; ++r) v[b * R0 + r] = ld(s, j + r * (L / R0));
     }
@@ -405,9 +441,9 @@ struct BlockFFT {
   __device__ __forceinline__ static void store(const cx<T>* v, int t, St st) {
 #pragma unroll
     for (int b = 0; b < E / RL; ++b) {
-      const int g = t + NT * b;
-      const int s = g % TS;
-      const int j = g / TS;
+      const unsigned g = (unsigned)t + NT * b;
+      const unsigned s = g % TS;
+      const unsigned j = g / TS;
 #pragma unroll
       for (int r = 0; r < RL; ++r) st(s, j + r * (L / RL), v[b * RL + r]);
     }
@@ -441,9 +477,14 @@ enum { MAP_STD = 0, MAP_SEQ = 1, MAP_PAIR = 2, MAP_SEQ1 = 3 };
 template <int MAP, int TS, int NT, int L, int R, int G>
 __device__ __forceinline__ void bmap(unsigned t, unsigned b, unsigned& s, unsigned& j) {
   if (MAP == MAP_STD) {
-    const unsigned g = t + NT * b;
-    s = g % TS;
-    j = g / TS;
+    if (NT % TS == 0) {  // the same sequence for every butterfly of a thread
+      s = t % TS;
+      j = t / TS + (unsigned)(NT / TS) * b;
+    } else {
+      const unsigned g = t + NT * b;
+      s = g % TS;
+      j = g / TS;
+    }
   } else if (MAP == MAP_SEQ || MAP == MAP_SEQ1) {
     constexpr unsigned GG = MAP == MAP_SEQ1 ? 1 : G;
     const unsigned q = b / GG, gg = b % GG;
@@ -522,6 +563,7 @@ struct BlockFFT2 {
     asm volatile("" : "+s"(tw));
     unsigned t = t_in;
     asm volatile("" : "+v"(t));
+    __builtin_assume(t < (unsigned)NT);
 #pragma unroll
     for (int b = 0; b < E / R; ++b) {
       cx<T>* w = v + b * R;
@@ -560,6 +602,7 @@ struct BlockFFT2 {
     // being hoisted (they are loop-invariant across transforms) and held live
     unsigned t = t_in;
     asm volatile("" : "+v"(t));
+    __builtin_assume(t < (unsigned)NT);
 #pragma unroll
     for (int c = 0; c < comps; ++c) {
       __syncthreads();
@@ -639,6 +682,7 @@ struct BlockFFT2 {
   __device__ __forceinline__ static void load_dft(cx<T>* v, int t_in, Ld ld) {
     unsigned t = t_in;
     asm volatile("" : "+v"(t));
+    __builtin_assume(t < (unsigned)NT);
 #pragma unroll
     for (int b = 0; b < E / R0; ++b) {
       unsigned s, j;

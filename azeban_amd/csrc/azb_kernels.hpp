@@ -712,6 +712,26 @@ __global__ void unpad_kernel(const double2* in, double2* out, int dim, int M, in
   }
 }
 
+// Streaming copy for the bandwidth reference: four independent 16-byte loads in
+// flight per thread per iteration, grid-stride over n 16-byte elements.
+__global__ __launch_bounds__(256) void stream_copy_kernel(const double2* __restrict__ x, double2* __restrict__ y,
+                                                          long n) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (; i + 3 * stride < n; i += 4 * stride) {
+    const double2 a = x[i], b = x[i + stride], c = x[i + 2 * stride], d = x[i + 3 * stride];
+    __builtin_nontemporal_store(a.x, &y[i].x);
+    __builtin_nontemporal_store(a.y, &y[i].y);
+    __builtin_nontemporal_store(b.x, &y[i + stride].x);
+    __builtin_nontemporal_store(b.y, &y[i + stride].y);
+    __builtin_nontemporal_store(c.x, &y[i + 2 * stride].x);
+    __builtin_nontemporal_store(c.y, &y[i + 2 * stride].y);
+    __builtin_nontemporal_store(d.x, &y[i + 3 * stride].x);
+    __builtin_nontemporal_store(d.y, &y[i + 3 * stride].y);
+  }
+  for (; i < n; i += stride) y[i] = x[i];
+}
+
 // x <- a x over a flat real array (scale.hpp)
 __global__ void scale_kernel(double a, double* x, long n) {
   const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -923,12 +943,13 @@ __global__ __launch_bounds__(P * L / 4) __attribute__((amdgpu_waves_per_eu(sizeo
   FI::run_rest(v, t, X, tw, typename PL::Inv{});
   // u[g*4 + r]: packed sequence type g of this thread's pair at its 4 positions
   cx<T> u[E];
-  double lmax = 0.0;
+  T lm = T(0);  // max|u| at working precision (exact), widened once
 #pragma unroll
   for (int e = 0; e < E; ++e) {
     u[e] = v[e];
-    lmax = fmax(lmax, (double)fmax(fabs(v[e].x), fabs(v[e].y)));
+    lm = fmax(lm, fmax(fabs(v[e].x), fabs(v[e].y)));
   }
+  double lmax = (double)lm;
   // settle the max here: left to itself the compiler sinks the reduction past the
   // product rounds and keeps its inputs alive (spilled) across them
   asm volatile("" : "+v"(lmax));

@@ -850,6 +850,45 @@ int azb_autotune(azb_solver* s, const void* u_hat, int reps) {
   return azb_set_tuning(s, pick, pick_row);
 }
 
+int azb_copy_bandwidth(double gib, int reps, double* gbs_out) {
+  if (!gbs_out || !(gib > 0.0) || gib > 64.0 || reps < 1) return fail(AZB_ERR_ARG, "bad arguments");
+  const long n = (long)(gib * 1073741824.0) / 16;
+  void *x = nullptr, *y = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = AZB_OK;
+  auto cleanup = [&]() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (x) (void)hipFree(x);
+    if (y) (void)hipFree(y);
+  };
+  if (hipMalloc(&x, n * 16) != hipSuccess || hipMalloc(&y, n * 16) != hipSuccess ||
+      hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    cleanup();
+    return fail(AZB_ERR_HIP, "allocation for the copy bandwidth probe failed");
+  }
+  int cus = 256;
+  int dev = 0;
+  if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const unsigned blocks = (unsigned)cus * 16;
+  if (hipMemset(x, 0x3f, n * 16) != hipSuccess) {
+    cleanup();
+    return fail(AZB_ERR_HIP, "copy bandwidth probe: fill failed");
+  }
+  hipLaunchKernelGGL(stream_copy_kernel, dim3(blocks), dim3(256), 0, nullptr, (const double2*)x, (double2*)y, n);
+  (void)hipEventRecord(e0, nullptr);
+  for (int r = 0; r < reps; ++r)
+    hipLaunchKernelGGL(stream_copy_kernel, dim3(blocks), dim3(256), 0, nullptr, (const double2*)x, (double2*)y, n);
+  (void)hipEventRecord(e1, nullptr);
+  float ms = 0.f;
+  if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess || ms <= 0.f)
+    rc = fail(AZB_ERR_HIP, "copy bandwidth probe failed");
+  else
+    *gbs_out = 2.0 * (double)n * 16.0 * reps / (ms * 1e-3) / 1e9;
+  cleanup();
+  return rc;
+}
+
 int azb_synchronize(azb_solver* s) {
   if (!s) return fail(AZB_ERR_ARG, "NULL solver");
   HIPCHK(hipStreamSynchronize(s->stream));

@@ -66,25 +66,15 @@ def counted_traffic(dim: int, N: int, family: str):
     return None
 
 
-def measured_copy_bandwidth(gib: float = 4.0, reps: int = 5) -> float:
-    """Achievable HBM bandwidth: a device-to-device copy (read + write) of `gib` GiB."""
-    import torch
+def measured_copy_bandwidth(gib: float = 4.0, reps: int = 10) -> float:
+    """Achievable HBM bandwidth: the library's streaming copy (read + write) of `gib` GiB."""
+    import ctypes as C
 
-    n = int(gib * 2**30) // 8
-    x = torch.ones(n, dtype=torch.float64, device="cuda")
-    y = torch.empty_like(x)
-    y.copy_(x)
-    torch.cuda.synchronize()
-    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    e0.record()
-    for _ in range(reps):
-        y.copy_(x)
-    e1.record()
-    torch.cuda.synchronize()
-    ms = e0.elapsed_time(e1) / reps
-    del x, y
-    torch.cuda.empty_cache()
-    return 2 * n * 8 / (ms * 1e-3) / 1e9
+    from azeban_amd._capi import check, load
+
+    out = C.c_double()
+    check(load().azb_copy_bandwidth(gib, reps, C.byref(out)))
+    return out.value
 
 
 def run_precision(A, dim, N, u_hat0, precision, steps, warmup):

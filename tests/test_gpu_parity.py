@@ -304,3 +304,15 @@ def test_fp32_mixed_error(cuda, dim, N, steps):
     err = rel(host(sim.u()), osim.u)
     print(f"fp32_mixed rel L2 vs FP64 oracle after {steps} steps: {err:.3e}")
     assert err <= 1e-5
+
+
+def test_copy_bandwidth_probe(cuda):
+    """The streaming-copy reference used for the measured roofline returns a sane rate."""
+    import ctypes as C
+
+    from azeban_amd._capi import check, load
+
+    out = C.c_double()
+    check(load().azb_copy_bandwidth(1.0, 3, C.byref(out)))
+    print(f"stream copy: {out.value:.0f} GB/s")
+    assert 500.0 < out.value < 20000.0
