@@ -676,6 +676,32 @@ struct BlockFFT2 {
     passes<1, NP, R0, R1, Rs...>(v, t, lds, tw, Plan<R1, Rs...>{});
   }
 
+  // Continue a transform whose first pass (radix R0, Ns = 1) the caller computed
+  // and wrote to LDS itself (element (s, pos) at LY::at(s, pos), followed by a
+  // barrier): read pass 1's inputs with pass 1's mapping and run passes 1 ...
+  template <int R0, int R1, int... Rs>
+  __device__ __forceinline__ static void run_after_first(cx<T>* v, int t_in, T* lds, const cx<T>* tw,
+                                                         Plan<R0, R1, Rs...>) {
+    static_assert(plan_prod<R0, R1, Rs...>() == L, "plan must multiply to L");
+    constexpr int NP = PlanAt<R0, R1, Rs...>::count;
+    unsigned t = t_in;
+    asm volatile("" : "+v"(t));
+    __builtin_assume(t < (unsigned)NT);
+#pragma unroll
+    for (int b = 0; b < E / R1; ++b) {
+      unsigned s, j;
+      bmap<map_of<1, NP>(), TS, NT, L, R1, G>(t, b, s, j);
+      const unsigned a0 = LY::at(s, j);
+#pragma unroll
+      for (int r = 0; r < R1; ++r) {
+        const unsigned idx = LY::linear(L / R1) ? a0 + r * LY::delta(L / R1) : LY::at(s, j + r * (L / R1));
+        v[b * R1 + r].x = lds[2 * idx];
+        v[b * R1 + r].y = lds[2 * idx + 1];
+      }
+    }
+    passes<1, NP, R0, R1, Rs...>(v, t_in, lds, tw, Plan<R1, Rs...>{});
+  }
+
   // first-pass load fused with the first pass's butterflies (which need no
   // twiddles), one butterfly group at a time, so only one group's loads are live
   template <int R0, class Ld>

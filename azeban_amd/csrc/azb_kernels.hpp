@@ -713,23 +713,23 @@ __global__ void unpad_kernel(const double2* in, double2* out, int dim, int M, in
 }
 
 // Streaming copy for the bandwidth reference: four independent 16-byte loads in
-// flight per thread per iteration, grid-stride over n 16-byte elements.
+// flight per thread per iteration, 16-byte non-temporal stores, grid-stride over
+// n 16-byte elements.
 __global__ __launch_bounds__(256) void stream_copy_kernel(const double2* __restrict__ x, double2* __restrict__ y,
                                                           long n) {
+  using v2 = vec2_t<double>;
+  const v2* __restrict__ xs = (const v2*)x;
+  v2* __restrict__ ys = (v2*)y;
   const long stride = (long)gridDim.x * blockDim.x;
   long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
   for (; i + 3 * stride < n; i += 4 * stride) {
-    const double2 a = x[i], b = x[i + stride], c = x[i + 2 * stride], d = x[i + 3 * stride];
-    __builtin_nontemporal_store(a.x, &y[i].x);
-    __builtin_nontemporal_store(a.y, &y[i].y);
-    __builtin_nontemporal_store(b.x, &y[i + stride].x);
-    __builtin_nontemporal_store(b.y, &y[i + stride].y);
-    __builtin_nontemporal_store(c.x, &y[i + 2 * stride].x);
-    __builtin_nontemporal_store(c.y, &y[i + 2 * stride].y);
-    __builtin_nontemporal_store(d.x, &y[i + 3 * stride].x);
-    __builtin_nontemporal_store(d.y, &y[i + 3 * stride].y);
+    const v2 a = xs[i], b = xs[i + stride], c = xs[i + 2 * stride], d = xs[i + 3 * stride];
+    __builtin_nontemporal_store(a, ys + i);
+    __builtin_nontemporal_store(b, ys + i + stride);
+    __builtin_nontemporal_store(c, ys + i + 2 * stride);
+    __builtin_nontemporal_store(d, ys + i + 3 * stride);
   }
-  for (; i < n; i += stride) y[i] = x[i];
+  for (; i < n; i += stride) __builtin_nontemporal_store(xs[i], ys + i);
 }
 
 // x <- a x over a flat real array (scale.hpp)
@@ -858,6 +858,77 @@ namespace azb {
 //              values a thread needs for separation come over lane permutes, and
 //              the k <= N/2 outputs leave as contiguous 16-byte stores.
 // ---------------------------------------------------------------------------
+// Packed first inverse pass of a row kernel, from HBM: radix 12 with one butterfly
+// per thread, butterfly j (< NB = L/12) of a sequence X + iY at positions j + NB r,
+// X and Y the half spectra (N/2 + 1 modes) of two real lines.  Positions up to
+// N/2 = 4 NB read k = pos (r < 4; r = 4 only for j = 0), positions from L - N/2 = 8 NB
+// read the conjugates at k = L - pos (r >= 8), the rest are the zero padding; the
+// k = 0 entry drops Im as the reference's c2r does.  Ends with the radix-12 DFT.
+template <typename T, int NB>
+__device__ __forceinline__ void row_first_pass(cx<T>* v, const cx<T>* __restrict__ px,
+                                               const cx<T>* __restrict__ py, unsigned j) {
+  const bool j0 = j == 0;
+#pragma unroll
+  for (int r = 0; r < 12; ++r) {
+    if (r < 4) {
+      const cx<T> X0 = px[j + NB * r], Y0 = py[j + NB * r];
+      v[r] = mk(X0.x - Y0.y, X0.y + Y0.x);
+      if (r == 0 && j0) v[r] = mk(X0.x, Y0.x);
+    } else if (r >= 8) {
+      const unsigned kk = NB * (12 - r) - j;
+      const cx<T> X0 = px[kk], Y0 = py[kk];
+      v[r] = mk(X0.x + Y0.y, Y0.x - X0.y);
+    } else {
+      v[r] = mk(T(0), T(0));
+      if (r == 4 && j0) {
+        const cx<T> X0 = px[4 * NB], Y0 = py[4 * NB];
+        v[r] = mk(X0.x - Y0.y, X0.y + Y0.x);
+      }
+    }
+  }
+  DFT<12, +1, T>::run(v);
+}
+
+// Separation of a packed forward transform Z = A + iB (A, B spectra of two real
+// lines) after a last pass of radix 12 with butterfly j of a sequence in lane j of
+// its wave (NB = L/12 dividing 64).  The thread holds Z[j + NB r], r < 12; the
+// mirror of position j + NB r is (NB - j) + NB (11 - r), index 11 - r of the lane
+// holding butterfly NB - j (for j = 0: index 12 - r of this lane).  Stores
+// A[k], B[k] for the k <= N/2 = 4 NB this lane owns (r < 4, and r = 4 for j = 0) as
+// contiguous 16-byte non-temporal stores.
+template <typename T, int NB>
+__device__ __forceinline__ void row_separate_store(const cx<T>* v, unsigned lane, unsigned j, vec2_t<T>* oa,
+                                                   vec2_t<T>* ob) {
+  const int mlane = (int)(lane - j + (NB - j) % NB);
+  cx<T> f[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    f[q].x = __shfl(v[11 - q].x, mlane);
+    f[q].y = __shfl(v[11 - q].y, mlane);
+  }
+  auto emit = [&](unsigned pos, const cx<T>& zk, const cx<T>& zm) {
+    vec2_t<T> va, vb;
+    va.x = T(0.5) * (zk.x + zm.x);
+    va.y = T(0.5) * (zk.y - zm.y);
+    vb.x = T(0.5) * (zk.y + zm.y);
+    vb.y = T(-0.5) * (zk.x - zm.x);
+    __builtin_nontemporal_store(va, oa + pos);
+    __builtin_nontemporal_store(vb, ob + pos);
+  };
+  const bool j0 = j == 0;
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    cx<T> zm = f[r];
+    if (r == 0) {
+      if (j0) zm = v[0];
+    } else {
+      if (j0) zm = f[r - 1];
+    }
+    emit(j + NB * r, v[r], zm);
+  }
+  if (j0) emit(4 * NB, v[4], f[3]);
+}
+
 template <int L>
 struct Row4Plan;
 template <> struct Row4Plan<768> { using Inv = Plan<12, 4, 4, 4>; using Fwd = Plan<4, 4, 4, 12>; };
@@ -902,12 +973,8 @@ __global__ __launch_bounds__(P * L / 4) __attribute__((amdgpu_waves_per_eu(sizeo
   const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
   const T nrm = T(a.norm);
 
-  // First inverse pass, loaded straight from HBM: radix 12 with one butterfly per
-  // thread (MAP_SEQ1), butterfly j of sequence s at positions j + NB r.  Sequence s
-  // is type s % 3 of pair s / 3: (u1a + i u2a, u3a + i u3b, u1b + i u2b), packed from
-  // the half spectra X, Y of its two real lines.  Positions up to N/2 = 4 NB read X, Y
-  // at k = pos (r < 4; r = 4 only for j = 0), positions from L - N/2 = 8 NB read the
-  // conjugates at k = L - pos (r >= 8), the rest are the zero padding.
+  // first inverse pass from HBM; sequence s is type s % 3 of pair s / 3:
+  // (u1a + i u2a, u3a + i u3b, u1b + i u2b)
   cx<T> v[E];
   {
     unsigned tt = t;
@@ -917,28 +984,7 @@ __global__ __launch_bounds__(P * L / 4) __attribute__((amdgpu_waves_per_eu(sizeo
     const long la = line0 + 2 * pair, lb = la + 1;
     const long lx = type == 0 ? la : (type == 1 ? 2 * a.nlines + la : lb);
     const long ly = type == 0 ? a.nlines + la : (type == 1 ? 2 * a.nlines + lb : a.nlines + lb);
-    const cx<T>* __restrict__ px = in + lx * a.ls;
-    const cx<T>* __restrict__ py = in + ly * a.ls;
-    const bool j0 = j == 0;
-#pragma unroll
-    for (int r = 0; r < 12; ++r) {
-      if (r < 4) {
-        const cx<T> X0 = px[j + NB * r], Y0 = py[j + NB * r];
-        v[r] = mk(X0.x - Y0.y, X0.y + Y0.x);
-        if (r == 0 && j0) v[r] = mk(X0.x, Y0.x);  // k = 0: c2r ignores Im
-      } else if (r >= 8) {
-        const unsigned kk = NB * (12 - r) - j;
-        const cx<T> X0 = px[kk], Y0 = py[kk];
-        v[r] = mk(X0.x + Y0.y, Y0.x - X0.y);
-      } else {
-        v[r] = mk(T(0), T(0));
-        if (r == 4 && j0) {
-          const cx<T> X0 = px[4 * NB], Y0 = py[4 * NB];
-          v[r] = mk(X0.x - Y0.y, X0.y + Y0.x);
-        }
-      }
-    }
-    DFT<12, +1, T>::run(v);
+    row_first_pass<T, NB>(v, in + lx * a.ls, in + ly * a.ls, j);
   }
   FI::run_rest(v, t, X, tw, typename PL::Inv{});
   // u[g*4 + r]: packed sequence type g of this thread's pair at its 4 positions
@@ -973,47 +1019,14 @@ __global__ __launch_bounds__(P * L / 4) __attribute__((amdgpu_waves_per_eu(sizeo
       }
     }
     FF::run(v, t, X, tw, typename PL::Fwd{});
-    // separation.  This thread holds Z[j + NB r], r < 12, of sequence s.  The mirror
-    // of position j + NB r is (NB - j) + NB (11 - r): index 11 - r of the lane that
-    // holds butterfly NB - j (for j = 0: index 12 - r of this lane).  Outputs are
-    // the positions k <= N/2 = 4 NB: r < 4, and r = 4 for j = 0.
     {
-      int tt = t;
+      unsigned tt = t;
       asm volatile("" : "+v"(tt));
-      const int s = tt / NB, j = tt % NB;
-      const int pair = s / 3, m = round * 3 + s % 3;
+      const unsigned s = tt / NB, j = tt % NB;
+      const unsigned pair = s / 3, m = round * 3 + s % 3;
       const long la = line0 + 2 * pair, lb = la + 1;
-      vec2_t<T>* oa = (vec2_t<T>*)(out + ((long)m * a.nlines + la) * a.ls);
-      vec2_t<T>* ob = (vec2_t<T>*)(out + ((long)m * a.nlines + lb) * a.ls);
-      const int lane = tt & 63;
-      const int mlane = lane - j + (NB - j) % NB;
-      cx<T> f[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        f[q].x = __shfl(v[11 - q].x, mlane);
-        f[q].y = __shfl(v[11 - q].y, mlane);
-      }
-      auto emit = [&](int pos, const cx<T>& zk, const cx<T>& zm) {
-        vec2_t<T> va, vb;
-        va.x = T(0.5) * (zk.x + zm.x);
-        va.y = T(0.5) * (zk.y - zm.y);
-        vb.x = T(0.5) * (zk.y + zm.y);
-        vb.y = T(-0.5) * (zk.x - zm.x);
-        __builtin_nontemporal_store(va, oa + pos);
-        __builtin_nontemporal_store(vb, ob + pos);
-      };
-      const bool j0 = j == 0;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        cx<T> zm = f[r];
-        if (r == 0) {
-          if (j0) zm = v[0];
-        } else {
-          if (j0) zm = f[r - 1];
-        }
-        emit(j + NB * r, v[r], zm);
-      }
-      if (j0) emit(4 * NB, v[4], f[3]);
+      row_separate_store<T, NB>(v, tt & 63, j, (vec2_t<T>*)(out + ((long)m * a.nlines + la) * a.ls),
+                                (vec2_t<T>*)(out + ((long)m * a.nlines + lb) * a.ls));
     }
   }
 
@@ -1023,6 +1036,117 @@ __global__ __launch_bounds__(P * L / 4) __attribute__((amdgpu_waves_per_eu(sizeo
   if (t == 0) {
     double mx = 0.0;
     for (int w = 0; w < NT / 64; ++w) mx = fmax(mx, red[w]);
+    atomicMax(&st->umax_bins[a.slot][blockIdx.x & (kBins - 1)],
+              (unsigned long long)__double_as_longlong(mx));
+  }
+}
+
+
+// ---------------------------------------------------------------------------
+// 2D row kernel (L <= 768), the 3D kernel's layout with one product round.
+// Per workgroup: P = 768/L line pairs, 192 threads.  Inverse: 2P packed sequences
+// (u1 + i u2 of each line, plan 12 ...): the radix-12 first pass (loaded from HBM)
+// runs on the first 2P * L/12 = 128 threads, one butterfly each, the later passes
+// on all threads with 8 values each; the last pass (radix 4, MAP_SEQ) leaves u1, u2
+// of both lines of a pair at the same four positions in one thread.  Products
+// (B11, B12, B22) packed (line a) + i (line b) feed the forward transform (3P
+// sequences, 12 values per thread) at those positions; its radix-12 last pass
+// separates the two half spectra over lane permutes as in the 3D kernel.
+// ---------------------------------------------------------------------------
+template <int L>
+struct Row2Plan;
+template <> struct Row2Plan<768> { using Inv = Plan<12, 4, 4, 4>; using Fwd = Plan<4, 4, 4, 12>; };
+template <> struct Row2Plan<384> { using Inv = Plan<12, 8, 4>; using Fwd = Plan<4, 4, 2, 12>; };
+template <> struct Row2Plan<192> { using Inv = Plan<12, 4, 4>; using Fwd = Plan<4, 4, 12>; };
+template <> struct Row2Plan<96> { using Inv = Plan<12, 2, 4>; using Fwd = Plan<4, 2, 12>; };
+
+template <typename T, int L>
+struct Row2Cfg {
+  static constexpr int P = 768 / L;
+  static constexpr int NT = P * L / 4;
+  static constexpr int TSI = 2 * P, TSF = 3 * P;
+  static constexpr int NB = L / 12;
+  static constexpr int SOFF = 11;
+  static constexpr int XWORDS = 2 * ExLayout<T, L, TSF, true, SOFF>::elems;
+};
+
+template <typename T, int L>
+__global__ __launch_bounds__((Row2Cfg<T, L>::NT)) void euler_row2_kernel(RowArgs a) {
+  using C = Row2Cfg<T, L>;
+  using PL = Row2Plan<L>;
+  constexpr int NT = C::NT, P = C::P, NB = C::NB, TSI = C::TSI, TSF = C::TSF;
+  using FI = BlockFFT2<T, L, TSI, NT, +1, false, 2, MAP_STD, MAP_SEQ, true, C::SOFF>;
+  using FF = BlockFFT2<T, L, TSF, NT, -1, false, 3, MAP_SEQ, MAP_SEQ1, true, C::SOFF>;
+  using LY = ExLayout<T, L, TSF, true, C::SOFF>;
+  static_assert(FI::E == 8 && FF::E == 12 && NT == 192, "thread layout");
+  static_assert(64 % NB == 0 && TSI * NB % 64 == 0, "whole waves in the radix-12 passes");
+  static_assert(first_radix(typename PL::Inv{}) == 12 && last_radix(typename PL::Inv{}) == 4 &&
+                    first_radix(typename PL::Fwd{}) == 4 && last_radix(typename PL::Fwd{}) == 12,
+                "plans must start/end on the radices the mappings assume");
+  __shared__ T X[C::XWORDS];
+  __shared__ double red[NT / 64];
+  const long bps = a.nlines / (2 * P);  // workgroups per sample
+  const long sample = blockIdx.x / bps;
+  DevState* st = a.st + sample;
+  if (st->done) return;
+
+  const int t = threadIdx.x;
+  const long line0 = (blockIdx.x % bps) * (2 * P);
+  const cx<T>* __restrict__ in = (const cx<T>*)a.in + sample * a.in_ss;
+  cx<T>* __restrict__ out = (cx<T>*)a.out + sample * a.out_ss;
+  const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
+  const T nrm = T(a.norm);
+
+  cx<T> v[12];
+  {
+    unsigned tt = t;
+    asm volatile("" : "+v"(tt));
+    if (tt < (unsigned)(TSI * NB)) {
+      const unsigned s = tt / NB, j = tt % NB;  // sequence s: u1 + i u2 of line line0 + s
+      const long l = line0 + s;
+      row_first_pass<T, NB>(v, in + l * a.ls, in + (a.nlines + l) * a.ls, j);
+      // Stockham pass-0 outputs of butterfly j sit at positions 12 j + r
+#pragma unroll
+      for (int r = 0; r < 12; ++r) {
+        const unsigned idx = LY::at(s, 12 * j + r);
+        X[2 * idx] = v[r].x;
+        X[2 * idx + 1] = v[r].y;
+      }
+    }
+  }
+  __syncthreads();
+  FI::run_after_first(v, t, X, tw, typename PL::Inv{});
+
+  // v[g*4 + r]: u1 + i u2 of line g of this thread's pair at its 4 positions
+  T lm = T(0);
+  cx<T> w[12];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const cx<T> pa = v[r], pb = v[4 + r];
+    lm = fmax(lm, fmax(fmax(fabs(pa.x), fabs(pa.y)), fmax(fabs(pb.x), fabs(pb.y))));
+    const T a1 = nrm * pa.x, a2 = nrm * pa.y, b1 = nrm * pb.x, b2 = nrm * pb.y;
+    w[r] = mk(a1 * a1, b1 * b1);      // B11
+    w[4 + r] = mk(a1 * a2, b1 * b2);  // B12
+    w[8 + r] = mk(a2 * a2, b2 * b2);  // B22
+  }
+  double lmax = (double)lm;
+  FF::run(w, t, X, tw, typename PL::Fwd{});
+  {
+    unsigned tt = t;
+    asm volatile("" : "+v"(tt));
+    const unsigned s = tt / NB, j = tt % NB;
+    const unsigned pair = s / 3, m = s % 3;
+    const long la = line0 + 2 * pair, lb = la + 1;
+    row_separate_store<T, NB>(w, tt & 63, j, (vec2_t<T>*)(out + ((long)m * a.nlines + la) * a.ls),
+                              (vec2_t<T>*)(out + ((long)m * a.nlines + lb) * a.ls));
+  }
+
+  lmax = wave_max<T>(lmax);
+  if ((t & 63) == 0) red[t >> 6] = lmax;
+  __syncthreads();
+  if (t == 0) {
+    double mx = 0.0;
+    for (int q = 0; q < NT / 64; ++q) mx = fmax(mx, red[q]);
     atomicMax(&st->umax_bins[a.slot][blockIdx.x & (kBins - 1)],
               (unsigned long long)__double_as_longlong(mx));
   }

@@ -149,6 +149,11 @@ void launch_row_L(RowArgs a, long nsamples, hipStream_t st, int pdiv) {
       launch_row4_P<T, L, P1>(a, nsamples, st);
     else
       launch_row4_P<T, L, P0>(a, nsamples, st);
+  } else if constexpr (L <= 768) {
+    // one shape: 768/L line pairs, 192 threads
+    const long blocks = a.nlines / (2 * Row2Cfg<T, L>::P) * nsamples;
+    hipLaunchKernelGGL((euler_row2_kernel<T, L>), dim3((unsigned)blocks), dim3(Row2Cfg<T, L>::NT), 0, st, a);
+    (void)pdiv;
   } else {
     if (pdiv >= 2)
       launch_row_P<T, L, P1, DIM>(a, nsamples, st);

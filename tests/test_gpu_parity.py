@@ -143,7 +143,7 @@ def make_pair(dim, N, precision="fp64"):
     return g, eq, oeq
 
 
-@pytest.mark.parametrize("dim,N", [(2, 64), (2, 128), (2, 256), (2, 1024), (3, 64), (3, 128)])
+@pytest.mark.parametrize("dim,N", [(2, 64), (2, 128), (2, 256), (2, 512), (2, 1024), (3, 64), (3, 128)])
 def test_dudt_fp64(cuda, dim, N):
     uh = random_state(dim, N)
     g, eq, oeq = make_pair(dim, N)
