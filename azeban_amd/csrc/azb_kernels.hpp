@@ -192,23 +192,6 @@ struct RowArgs {
   int slot;         // umax slot
 };
 
-template <typename T, int L, int P, int DIM>
-struct RowCfg {
-  static constexpr int E = LenPlan<L>::E;
-  static constexpr int NINV = DIM == 3 ? 3 : 2;
-  static constexpr int NROUND = DIM == 3 ? 2 : 1;
-  static constexpr int TSI = NINV * P;
-  static constexpr int TSF = 3 * P;
-  static constexpr int NTI = TSI * L / E;
-  static constexpr int NTF = TSF * L / E;
-  static constexpr int NT = NTI > NTF ? NTI : NTF;
-  static constexpr int N = 2 * L / 3;
-  static constexpr int NH = N / 2 + 1;
-  static constexpr bool SPLIT = sizeof(T) == 8;
-  static constexpr int XW = (TSI > TSF ? TSI : TSF) * L;  // exchange words (split layout)
-  static constexpr int XWORDS = SPLIT ? XW : 2 * XW;
-};
-
 template <typename T>
 __device__ __forceinline__ double wave_max(double v) {
 #pragma unroll
@@ -216,183 +199,6 @@ __device__ __forceinline__ double wave_max(double v) {
   return v;
 }
 
-template <typename T, int L, int P, int DIM>
-__global__ __launch_bounds__((RowCfg<T, L, P, DIM>::NT)) void euler_row_kernel(RowArgs a) {
-  using C = RowCfg<T, L, P, DIM>;
-  using LP = LenPlan<L>;
-  constexpr int NT = C::NT, E = C::E, NH = C::NH, N = C::N;
-  constexpr int R0 = first_radix(typename LP::P{});
-  constexpr int RL = last_radix(typename LP::P{});
-  using FI = BlockFFT<T, L, C::TSI, C::NTI, +1, C::SPLIT>;
-  using FF = BlockFFT<T, L, C::TSF, C::NTF, -1, C::SPLIT>;
-  __shared__ cx<T> U[C::TSI * L];
-  __shared__ T X[C::XWORDS];
-  __shared__ double red[NT / 64 + 1];
-  const long bps = a.nlines / (2 * P);  // workgroups per sample
-  const long sample = blockIdx.x / bps;
-  DevState* st = a.st + sample;
-  if (st->done) return;
-
-  const int t = threadIdx.x;
-  const long line0 = (blockIdx.x % bps) * (2 * P);
-  const cx<T>* __restrict__ in = (const cx<T>*)a.in + sample * a.in_ss;
-  cx<T>* __restrict__ out = (cx<T>*)a.out + sample * a.out_ss;
-  const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
-  const T nrm = T(a.norm);
-
-  // ---- stage the inputs: stg[(c*2P + l)*NH + k]
-  {
-    cx<T>* stg = U;
-    constexpr int total = DIM * 2 * P * NH;
-    for (int e = t; e < total; e += NT) {
-      const int k = e % NH;
-      const int cl = e / NH;
-      const int l = cl % (2 * P);
-      const int c = cl / (2 * P);
-      stg[e] = in[((long)c * a.nlines + line0 + l) * a.ls + k];
-    }
-  }
-  __syncthreads();
-
-  // ---- inverse: packed Hermitian-extended lines
-  cx<T> v[E];
-  const bool act_i = t < C::NTI;
-  if (act_i) {
-    FI::template load<R0>(v, t, [&](int s, int pos) -> cx<T> {
-      const int pair = s / C::NINV, type = s % C::NINV;
-      const int la = 2 * pair, lb = 2 * pair + 1;
-      int cx_c, cx_l, cy_c, cy_l;
-      if (DIM == 3) {
-        if (type == 0) { cx_c = 0; cx_l = la; cy_c = 1; cy_l = la; }
-        else if (type == 1) { cx_c = 2; cx_l = la; cy_c = 2; cy_l = lb; }
-        else { cx_c = 0; cx_l = lb; cy_c = 1; cy_l = lb; }
-      } else {
-        if (type == 0) { cx_c = 0; cx_l = la; cy_c = 1; cy_l = la; }
-        else { cx_c = 0; cx_l = lb; cy_c = 1; cy_l = lb; }
-      }
-      const cx<T>* xs = U + (cx_c * 2 * P + cx_l) * NH;
-      const cx<T>* ys = U + (cy_c * 2 * P + cy_l) * NH;
-      if (pos == 0) return mk(xs[0].x, ys[0].x);
-      if (pos <= N / 2) {
-        const cx<T> X0 = xs[pos], Y0 = ys[pos];
-        return mk(X0.x - Y0.y, X0.y + Y0.x);
-      }
-      if (pos >= L - N / 2) {
-        const cx<T> X0 = xs[L - pos], Y0 = ys[L - pos];
-        return mk(X0.x + Y0.y, Y0.x - X0.y);
-      }
-      return mk(T(0), T(0));
-    });
-  }
-  __syncthreads();  // all staging reads done before U is overwritten
-  FI::run(v, t, X, tw, act_i, typename LP::P{});
-  if (act_i) {
-    FI::template store<RL>(v, t, [&](int s, int pos, cx<T> val) { U[s * L + pos] = val; });
-  }
-  __syncthreads();
-
-  double lmax = 0.0;
-#pragma unroll
-  for (int round = 0; round < C::NROUND; ++round) {
-    const bool act_f = t < C::NTF;
-    if (act_f) {
-      FF::template load<R0>(v, t, [&](int s, int pos) -> cx<T> {
-        const int pair = s / 3, type = s % 3;
-        T ua1, ua2, ub1, ub2, ua3 = 0, ub3 = 0;
-        if (DIM == 3) {
-          const cx<T> p0 = U[(pair * 3 + 0) * L + pos];
-          const cx<T> p1 = U[(pair * 3 + 1) * L + pos];
-          const cx<T> p2 = U[(pair * 3 + 2) * L + pos];
-          ua1 = p0.x; ua2 = p0.y; ua3 = p1.x; ub3 = p1.y; ub1 = p2.x; ub2 = p2.y;
-        } else {
-          const cx<T> p0 = U[(pair * 2 + 0) * L + pos];
-          const cx<T> p1 = U[(pair * 2 + 1) * L + pos];
-          ua1 = p0.x; ua2 = p0.y; ub1 = p1.x; ub2 = p1.y;
-        }
-        if (round == 0 && type == 1) {
-          lmax = fmax(lmax, (double)fmax(fmax(fabs(ua1), fabs(ua2)), fmax(fabs(ub1), fabs(ub2))));
-        }
-        if (DIM == 3 && round == 1 && type == 2) {
-          lmax = fmax(lmax, (double)fmax(fabs(ua3), fabs(ub3)));
-        }
-        const T a1 = nrm * ua1, a2 = nrm * ua2, a3 = nrm * ua3;
-        const T b1 = nrm * ub1, b2 = nrm * ub2, b3 = nrm * ub3;
-        const int m = round * 3 + type;
-        if (DIM == 3) {
-          // (11, 21, 22, 31, 32, 33) = (u1u1, u2u1, u2u2, u3u1, u3u2, u3u3)
-          switch (m) {
-            case 0: return mk(a1 * a1, b1 * b1);
-            case 1: return mk(a2 * a1, b2 * b1);
-            case 2: return mk(a2 * a2, b2 * b2);
-            case 3: return mk(a3 * a1, b3 * b1);
-            case 4: return mk(a3 * a2, b3 * b2);
-            default: return mk(a3 * a3, b3 * b3);
-          }
-        } else {
-          switch (m) {
-            case 0: return mk(a1 * a1, b1 * b1);
-            case 1: return mk(a1 * a2, b1 * b2);
-            default: return mk(a2 * a2, b2 * b2);
-          }
-        }
-      });
-    }
-    FF::run(v, t, X, tw, act_f, typename LP::P{});
-    // separate the two half spectra: A = (Z[k] + conj Z[-k]) / 2, B = (Z[k] - conj Z[-k]) / 2i
-    constexpr int items = C::TSF * NH;
-    constexpr int per = (items + NT - 1) / NT;
-    T ra[per], rb[per], ia[per], ib[per];
-#pragma unroll
-    for (int ph = 0; ph < 2; ++ph) {
-      __syncthreads();
-      if (act_f) {
-        FF::template store<RL>(v, t, [&](int s, int pos, cx<T> val) {
-          X[s * L + pos] = ph == 0 ? val.x : val.y;
-        });
-      }
-      __syncthreads();
-#pragma unroll
-      for (int q = 0; q < per; ++q) {
-        const int it = t + NT * q;
-        if (it < items) {
-          const int k = it % NH, s = it / NH;
-          const T zk = X[s * L + k];
-          const T zm = X[s * L + (k == 0 ? 0 : L - k)];
-          if (ph == 0) {
-            ra[q] = T(0.5) * (zk + zm);
-            ib[q] = T(-0.5) * (zk - zm);
-          } else {
-            ia[q] = T(0.5) * (zk - zm);
-            rb[q] = T(0.5) * (zk + zm);
-          }
-        }
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < per; ++q) {
-      const int it = t + NT * q;
-      if (it < items) {
-        const int k = it % NH, s = it / NH;
-        const int pair = s / 3, type = s % 3;
-        const int m = round * 3 + type;
-        const long la = line0 + 2 * pair, lb = la + 1;
-        out[((long)m * a.nlines + la) * a.ls + k] = mk(ra[q], ia[q]);
-        out[((long)m * a.nlines + lb) * a.ls + k] = mk(rb[q], ib[q]);
-      }
-    }
-  }
-
-  // ---- block max -> one atomic per workgroup into a spread bin
-  lmax = wave_max<T>(lmax);
-  if ((t & 63) == 0) red[t >> 6] = lmax;
-  __syncthreads();
-  if (t == 0) {
-    double m = 0.0;
-    for (int w = 0; w < (NT + 63) / 64; ++w) m = fmax(m, red[w]);
-    atomicMax(&st->umax_bins[a.slot][blockIdx.x & (kBins - 1)],
-              (unsigned long long)__double_as_longlong(m));
-  }
-}
 
 // ---------------------------------------------------------------------------
 // K-space kernel with the RK stage update fused.
@@ -891,7 +697,7 @@ __device__ __forceinline__ void row_first_pass(cx<T>* v, const cx<T>* __restrict
 
 // Separation of a packed forward transform Z = A + iB (A, B spectra of two real
 // lines) after a last pass of radix 12 with butterfly j of a sequence in lane j of
-// its wave (NB = L/12 dividing 64).  The thread holds Z[j + NB r], r < 12; the
+// its wave (NB = L/12 dividing 64) or in the MAP_MIRROR layout (NB a multiple of 64).  The thread holds Z[j + NB r], r < 12; the
 // mirror of position j + NB r is (NB - j) + NB (11 - r), index 11 - r of the lane
 // holding butterfly NB - j (for j = 0: index 12 - r of this lane).  Stores
 // A[k], B[k] for the k <= N/2 = 4 NB this lane owns (r < 4, and r = 4 for j = 0) as
@@ -899,7 +705,10 @@ __device__ __forceinline__ void row_first_pass(cx<T>* v, const cx<T>* __restrict
 template <typename T, int NB>
 __device__ __forceinline__ void row_separate_store(const cx<T>* v, unsigned lane, unsigned j, vec2_t<T>* oa,
                                                    vec2_t<T>* ob) {
-  const int mlane = (int)(lane - j + (NB - j) % NB);
+  // NB <= 64: butterfly j in lane j of a wave per sequence; NB > 64: MAP_MIRROR
+  // (mirror in lane ^ 32, the self-mirrored 0 and NB/2 in lanes 0 and 32)
+  const int mlane = NB <= 64 ? (int)(lane - j + (NB - j) % NB)
+                             : (int)((j == 0 || j == NB / 2) ? lane : (lane ^ 32u));
   cx<T> f[4];
 #pragma unroll
   for (int q = 0; q < 4; ++q) {
@@ -1059,10 +868,11 @@ template <> struct Row2Plan<768> { using Inv = Plan<12, 4, 4, 4>; using Fwd = Pl
 template <> struct Row2Plan<384> { using Inv = Plan<12, 8, 4>; using Fwd = Plan<4, 4, 2, 12>; };
 template <> struct Row2Plan<192> { using Inv = Plan<12, 4, 4>; using Fwd = Plan<4, 4, 12>; };
 template <> struct Row2Plan<96> { using Inv = Plan<12, 2, 4>; using Fwd = Plan<4, 2, 12>; };
+template <> struct Row2Plan<1536> { using Inv = Plan<12, 8, 4, 4>; using Fwd = Plan<4, 4, 4, 2, 12>; };
 
 template <typename T, int L>
 struct Row2Cfg {
-  static constexpr int P = 768 / L;
+  static constexpr int P = L <= 768 ? 768 / L : 1;
   static constexpr int NT = P * L / 4;
   static constexpr int TSI = 2 * P, TSF = 3 * P;
   static constexpr int NB = L / 12;
@@ -1076,10 +886,11 @@ __global__ __launch_bounds__((Row2Cfg<T, L>::NT)) void euler_row2_kernel(RowArgs
   using PL = Row2Plan<L>;
   constexpr int NT = C::NT, P = C::P, NB = C::NB, TSI = C::TSI, TSF = C::TSF;
   using FI = BlockFFT2<T, L, TSI, NT, +1, false, 2, MAP_STD, MAP_SEQ, true, C::SOFF>;
-  using FF = BlockFFT2<T, L, TSF, NT, -1, false, 3, MAP_SEQ, MAP_SEQ1, true, C::SOFF>;
+  constexpr int MAPL = NB <= 64 ? MAP_SEQ1 : MAP_MIRROR;
+  using FF = BlockFFT2<T, L, TSF, NT, -1, false, 3, MAP_SEQ, MAPL, true, C::SOFF>;
   using LY = ExLayout<T, L, TSF, true, C::SOFF>;
-  static_assert(FI::E == 8 && FF::E == 12 && NT == 192, "thread layout");
-  static_assert(64 % NB == 0 && TSI * NB % 64 == 0, "whole waves in the radix-12 passes");
+  static_assert(FI::E == 8 && FF::E == 12, "thread layout");
+  static_assert((64 % NB == 0 || NB % 64 == 0) && TSI * NB % 64 == 0, "whole waves in the radix-12 passes");
   static_assert(first_radix(typename PL::Inv{}) == 12 && last_radix(typename PL::Inv{}) == 4 &&
                     first_radix(typename PL::Fwd{}) == 4 && last_radix(typename PL::Fwd{}) == 12,
                 "plans must start/end on the radices the mappings assume");
@@ -1134,7 +945,8 @@ __global__ __launch_bounds__((Row2Cfg<T, L>::NT)) void euler_row2_kernel(RowArgs
   {
     unsigned tt = t;
     asm volatile("" : "+v"(tt));
-    const unsigned s = tt / NB, j = tt % NB;
+    unsigned s, j;
+    bmap<MAPL, TSF, NT, L, 12, 3>(tt, 0, s, j);
     const unsigned pair = s / 3, m = s % 3;
     const long la = line0 + 2 * pair, lb = la + 1;
     row_separate_store<T, NB>(w, tt & 63, j, (vec2_t<T>*)(out + ((long)m * a.nlines + la) * a.ls),

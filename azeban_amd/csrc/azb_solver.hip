@@ -125,14 +125,6 @@ int launch_col_full(int L, ColArgs a, long batches, hipStream_t st) {
   return AZB_OK;
 }
 
-template <typename T, int L, int P, int DIM>
-void launch_row_P(RowArgs a, long nsamples, hipStream_t st) {
-  using C = RowCfg<T, L, P, DIM>;
-  // every workgroup owns exactly P whole line pairs of one sample (checked by the caller)
-  const long blocks = a.nlines / (2 * P) * nsamples;
-  hipLaunchKernelGGL((euler_row_kernel<T, L, P, DIM>), dim3((unsigned)blocks), dim3(C::NT), 0, st, a);
-}
-
 template <typename T, int L, int P>
 void launch_row4_P(RowArgs a, long nsamples, hipStream_t st) {
   const long blocks = a.nlines / (2 * P) * nsamples;
@@ -149,16 +141,11 @@ void launch_row_L(RowArgs a, long nsamples, hipStream_t st, int pdiv) {
       launch_row4_P<T, L, P1>(a, nsamples, st);
     else
       launch_row4_P<T, L, P0>(a, nsamples, st);
-  } else if constexpr (L <= 768) {
-    // one shape: 768/L line pairs, 192 threads
+  } else {
+    // one shape: max(768/L, 1) line pairs, 192 threads (384 at L = 1536)
     const long blocks = a.nlines / (2 * Row2Cfg<T, L>::P) * nsamples;
     hipLaunchKernelGGL((euler_row2_kernel<T, L>), dim3((unsigned)blocks), dim3(Row2Cfg<T, L>::NT), 0, st, a);
     (void)pdiv;
-  } else {
-    if (pdiv >= 2)
-      launch_row_P<T, L, P1, DIM>(a, nsamples, st);
-    else
-      launch_row_P<T, L, P0, DIM>(a, nsamples, st);
   }
 }
 
