@@ -12,6 +12,11 @@ LIB_PATH = Path(__file__).resolve().parent / "libazeban_hip.so"
 
 AZB_FP64 = 0
 AZB_FP32_MIXED = 1
+AZB_OK = 0
+AZB_ERR_ARG = -1
+AZB_ERR_HIP = -2
+AZB_ERR_UNSUPPORTED = -3
+AZB_ERR_ALLOC = -4
 
 PROF_FAMILIES = ["inv_ax0", "inv_ax1", "row_fused", "fwd_ax1", "fwd_ax0", "kspace_rk"]
 

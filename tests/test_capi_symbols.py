@@ -35,3 +35,50 @@ def test_binding_covers_header():
 def test_load_sets_prototypes():
     lib = _capi.load()
     assert lib.azb_last_error() is not None
+
+
+# ---------------------------------------------------------------------------
+# Argument validation: these entry points reject bad input with a status code and
+# a message before any device call (the reference's LOG_ERR checks become codes)
+# ---------------------------------------------------------------------------
+import ctypes as C  # noqa: E402
+
+import pytest  # noqa: E402
+
+
+def _create(dim, N, M=0, prec=None, samples=1):
+    lib = _capi.load()
+    h = C.c_void_p()
+    visc = _capi.SVParams(0.05 / max(N, 1), 1.0, 1.0)
+    prec = _capi.AZB_FP64 if prec is None else prec
+    rc = lib.azb_create_batch(dim, N, M, dim, prec, visc, samples, None, C.byref(h))
+    return rc, h, lib.azb_last_error().decode()
+
+
+@pytest.mark.parametrize("dim,N,M,prec,samples,code,needle", [
+    (4, 64, 0, None, 1, -1, "dim"),
+    (3, 66, 0, None, 1, -1, "multiple of 4"),
+    (3, 64, 100, None, 1, -3, "3N/2"),
+    (3, 100, 0, None, 1, -3, "not built"),
+    (3, 1024, 0, None, 1, -3, "not built"),
+    (2, 64, 0, 7, 1, -1, "precision"),
+    (2, 64, 0, None, 0, -1, "nsamples"),
+])
+def test_create_rejects_bad_arguments(dim, N, M, prec, samples, code, needle):
+    rc, h, msg = _create(dim, N, M, prec, samples)
+    assert rc == code and not h.value
+    assert needle in msg
+
+
+def test_entry_points_reject_null_handles():
+    lib = _capi.load()
+    dt = C.c_double()
+    assert lib.azb_dudt(None, None, None) == _capi.AZB_ERR_ARG
+    assert lib.azb_rk3_step(None, None, 1.0, 0.2, C.byref(dt)) == _capi.AZB_ERR_ARG
+    assert lib.azb_dt(None, 0.2, C.byref(dt)) == _capi.AZB_ERR_ARG
+    assert lib.azb_set_tuning(None, None, 1) == _capi.AZB_ERR_ARG
+    assert lib.azb_copy_bandwidth(-1.0, 1, C.byref(dt)) == _capi.AZB_ERR_ARG
+    assert lib.azb_copy_from_padded(2, 24, 16, None, None, None) == _capi.AZB_ERR_ARG
+    assert lib.azb_copy_from_padded(4, 24, 16, C.c_void_p(8), C.c_void_p(8), None) == _capi.AZB_ERR_ARG
+    assert lib.azb_scale(1.0, None, 4, None) == _capi.AZB_ERR_ARG
+    assert "NULL" in lib.azb_last_error().decode() or lib.azb_last_error()

@@ -292,6 +292,31 @@ def test_simulate_until_device_loop_matches_host_loop(cuda):
     assert rel(host(a.u()), host(b.u())) <= 1e-14
 
 
+def test_simulate_until_edge_cases(cuda):
+    """Simulation::simulate_until (src/azeban/simulation.cpp:89-102): no step when the
+    target is not ahead, time_ = t afterwards even if t lies behind, a target closer
+    than one CFL step is reached in one clamped step, and max_steps bounds the loop."""
+    N = 64
+    uh = O.initialize(I.double_shear_layer(N, seed=4))
+    g, eq, _ = make_pair(2, N)
+    sim = A.Simulation(g, 0.2, A.SSP_RK3(g.shape_fourier(2), eq), dev(uh))
+    u0 = host(sim.u()).copy()
+    assert sim.simulate_until(0.0) == 0 and sim.time() == 0.0
+    assert np.array_equal(host(sim.u()), u0)
+    sim.set_time(0.5)
+    assert sim.simulate_until(0.25) == 0 and sim.time() == 0.25
+    assert np.array_equal(host(sim.u()), u0)
+    sim.reset()
+    t_small = 1e-5  # far below one CFL step at N = 64
+    g2, eq2, _ = make_pair(2, N)
+    ref = A.Simulation(g2, 0.2, A.SSP_RK3(g2.shape_fourier(2), eq2), dev(uh))
+    assert sim.simulate_until(t_small) == 1 and ref.simulate_until_host(t_small) == 1
+    assert sim.time() == t_small == ref.time()
+    assert rel(host(sim.u()), host(ref.u())) <= 1e-15
+    n = sim.simulate_until(1.0, max_steps=3, chunk=2)
+    assert n == 3 and t_small < sim.time() < 1.0
+
+
 @pytest.mark.parametrize("dim,N,steps", [(2, 256, 20), (3, 64, 10)])
 def test_fp32_mixed_error(cuda, dim, N, steps):
     print(f"[case] fp32_mixed dim={dim} N={N}", flush=True)
