@@ -461,9 +461,6 @@ namespace azb {
 //             after the last pass it holds the same positions of all G sequences
 //             (elementwise products across sequences need no exchange), and a first
 //             pass reading HBM is contiguous in j across threads
-//   MAP_PAIR: a thread owns the butterfly pair (j, L/R - j) of one sequence (with
-//             0 paired with L/(2R)), so every output position and its mirror L - pos
-//             sit in the same thread after the last pass
 //   MAP_SEQ1: MAP_SEQ with G = 1 whatever the transform's G: consecutive threads own
 //             consecutive butterflies of one sequence (a radix-L/64 pass puts each
 //             sequence in exactly one wave, butterfly j in lane j)
@@ -475,14 +472,14 @@ namespace azb {
 // Any mapping can be used by any pass: the LDS exchange writes with the mapping of
 // the pass that produced the data and reads with the mapping of the next pass.
 // ---------------------------------------------------------------------------
-enum { MAP_STD = 0, MAP_SEQ = 1, MAP_PAIR = 2, MAP_SEQ1 = 3, MAP_MIRROR = 4 };
+enum { MAP_STD = 0, MAP_SEQ = 1, MAP_SEQ1 = 3, MAP_MIRROR = 4 };
 
 // Index arithmetic is unsigned throughout: signed division and modulo by a power of
 // two cost a sign fix-up each, and these run once per element of every exchange.
 template <int MAP, int TS, int NT, int L, int R, int G>
 __device__ __forceinline__ void bmap(unsigned t, unsigned b, unsigned& s, unsigned& j) {
-  if (MAP == MAP_STD) {
-    if (NT % TS == 0) {  // the same sequence for every butterfly of a thread
+  if constexpr (MAP == MAP_STD) {
+    if constexpr (NT % TS == 0) {  // the same sequence for every butterfly of a thread
       s = t % TS;
       j = t / TS + (unsigned)(NT / TS) * b;
     } else {
@@ -490,13 +487,14 @@ __device__ __forceinline__ void bmap(unsigned t, unsigned b, unsigned& s, unsign
       s = g % TS;
       j = g / TS;
     }
-  } else if (MAP == MAP_SEQ || MAP == MAP_SEQ1) {
+  } else if constexpr (MAP == MAP_SEQ || MAP == MAP_SEQ1) {
     constexpr unsigned GG = MAP == MAP_SEQ1 ? 1 : G;
     const unsigned q = b / GG, gg = b % GG;
     const unsigned u = t + NT * q;
     j = u % (unsigned)(L / R);
     s = (u / (unsigned)(L / R)) * GG + gg;
-  } else if (MAP == MAP_MIRROR) {
+  } else {
+    static_assert(MAP == MAP_MIRROR, "unknown mapping");
     constexpr unsigned NB = L / R;
     const unsigned u = t + NT * b;
     s = u / NB;
@@ -505,13 +503,6 @@ __device__ __forceinline__ void bmap(unsigned t, unsigned b, unsigned& s, unsign
       j = 32 * w + l;
     else
       j = (w == 0 && l == 32) ? NB / 2 : NB - (32 * w + l - 32);
-  } else {
-    constexpr unsigned H = L / (2 * R);
-    const unsigned q = b / 2, e = b % 2;
-    const unsigned u = t + NT * q;
-    s = u / H;
-    const unsigned p = u % H;
-    j = e == 0 ? p : (p == 0 ? H : (unsigned)(L / R) - p);
   }
 }
 
