@@ -147,12 +147,14 @@ class Ensemble:
 
 
 def solve_one(dim: int, N: int, u_hat: torch.Tensor, t_end: float, C_: float = 0.2,
-              precision: str = "fp64", chunk: int = 16) -> tuple[torch.Tensor, int]:
-    """One sample alone through the single-solve path (azb_create + azb_simulate_until)."""
+              precision: str = "fp64", chunk: int = 16, eq=None) -> tuple[torch.Tensor, int]:
+    """One sample alone through the single-solve path (azb_create + azb_simulate_until);
+    `eq` reuses an existing IncompressibleEuler of that grid (its workspace and tuning)."""
     from .solver import IncompressibleEuler, Grid, SmoothCutoff1D, Simulation, SSP_RK3
 
     g = Grid(dim, N)
-    eq = IncompressibleEuler(g, SmoothCutoff1D(0.05 / N, 1.0, 2 * math.pi * N / 4), precision)
+    if eq is None:
+        eq = IncompressibleEuler(g, SmoothCutoff1D(0.05 / N, 1.0, 2 * math.pi * N / 4), precision)
     sim = Simulation(g, C_, SSP_RK3(g.shape_fourier(dim), eq), u_hat)
     n = sim.simulate_until(t_end, chunk=chunk)
     return sim.u(), n
