@@ -843,7 +843,7 @@ int azb_autotune(azb_solver* s, const void* u_hat, int reps) {
 }
 
 int azb_copy_bandwidth(double gib, int reps, double* gbs_out) {
-  if (!gbs_out || !(gib > 0.0) || gib > 64.0 || reps < 1) return fail(AZB_ERR_ARG, "bad arguments");
+  if (!gbs_out || !(gib > 0.0) || gib > 32.0 || reps < 1) return fail(AZB_ERR_ARG, "bad arguments");
   const long n = (long)(gib * 1073741824.0) / 16;
   void *x = nullptr, *y = nullptr;
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -859,10 +859,7 @@ int azb_copy_bandwidth(double gib, int reps, double* gbs_out) {
     cleanup();
     return fail(AZB_ERR_HIP, "allocation for the copy bandwidth probe failed");
   }
-  int cus = 256;
-  int dev = 0;
-  if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  const unsigned blocks = (unsigned)cus * 16;
+  const unsigned blocks = (unsigned)((n + 255) / 256);  // n < 2^31
   if (hipMemset(x, 0x3f, n * 16) != hipSuccess) {
     cleanup();
     return fail(AZB_ERR_HIP, "copy bandwidth probe: fill failed");

@@ -209,10 +209,11 @@ int azb_memcpy_d2h_async(void* dst_host, const void* src, size_t bytes, void* st
 int azb_synchronize(azb_solver* s);
 int azb_device_synchronize(void);
 
-/* Achievable HBM bandwidth on the current device: a streaming copy (16-byte loads,
- * non-temporal stores) between two fresh buffers of gib GiB, reps times, timed with
- * events; *gbs_out = bytes read + written per second / 1e9.  Synchronising.  No
- * counterpart in the reference: the measured roofline beside the 8 TB/s nominal one. */
+/* Achievable HBM bandwidth on the current device: a streaming copy (one 16-byte
+ * load and non-temporal store per thread) between two fresh buffers of gib GiB
+ * (0 < gib <= 32), reps times, timed with events; *gbs_out = bytes read + written
+ * per second / 1e9.  Synchronising.  No counterpart in the reference: the measured
+ * roofline beside the 8 TB/s nominal one. */
 int azb_copy_bandwidth(double gib, int reps, double* gbs_out);
 
 #ifdef __cplusplus
