@@ -265,6 +265,12 @@ def test_rk3_linear_closed_form():
     sim.simulate_until(1.0, max_steps=6)
     ratio = np.prod([1 + nu * dt + (nu * dt) ** 2 / 2 + (nu * dt) ** 3 / 6 for dt in sim.dts])
     assert sim.u[0, 0, m] == pytest.approx(uh[0, 0, m] * ratio, rel=1e-12)
+    # CFL step: u_max = max|u| / M^2 on the padded grid of the unnormalised transform,
+    # where sin(2 pi m j / M) reaches 1 (m = 20, M = 96, j = 6), so u_max = (N/M)^2
+    M = 3 * N // 2
+    assert sim.dts[0] == pytest.approx(0.2 / (N * (N / M) ** 2), rel=1e-14)
+    amp1 = 1 + nu * sim.dts[0] + (nu * sim.dts[0]) ** 2 / 2 + (nu * sim.dts[0]) ** 3 / 6
+    assert sim.dts[1] == pytest.approx(0.2 / (N * (N / M) ** 2 * amp1), rel=1e-13)
 
 
 def test_simulate_until_lands_on_t():
