@@ -258,6 +258,8 @@ def main():
     ap.add_argument("--max-active", type=int, default=32)
     ap.add_argument("--N", type=int, default=0)
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--error-steps", type=int, default=200,
+                    help="steps after which the FP32-mixed state is compared with FP64 (both continued)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--calib-copy-gib", type=float, default=0.0,
@@ -360,13 +362,19 @@ def main():
     if "fp32_mixed" in res:
         msm, kernm = summarize(res["fp32_mixed"], "fp32_mixed")
         vm = pts * args.steps / (res["fp32_mixed"]["gpu_ms"] * 1e-3) / 1e6
-        a = res["fp64"]["sim"].u()
-        b = res["fp32_mixed"]["sim"].u()
+        sa, sb = res["fp64"]["sim"], res["fp32_mixed"]["sim"]
+        done = args.warmup + 2 * args.steps
+        if args.error_steps > done:
+            log(f"continuing both precisions to {args.error_steps} steps for the error")
+            sa.simulate_until(1e9, max_steps=args.error_steps - done, chunk=16)
+            sb.simulate_until(1e9, max_steps=args.error_steps - done, chunk=16)
+        a, b = sa.u(), sb.u()
         err = float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(a))
         out["fp32_mixed"] = {"value": round(vm, 2), "ms_per_step": round(msm, 3),
                              "speedup_vs_fp64": round(vm / value, 3),
                              "rel_l2_vs_fp64_gpu": err,
-                             "steps_compared": args.warmup + 2 * args.steps,
+                             "steps_compared": max(done, args.error_steps),
+                             "time_fp64": sa.time(), "time_fp32_mixed": sb.time(),
                              "kernels": kernm}
     if not args.no_cpu_baseline:
         log("cpu baseline")

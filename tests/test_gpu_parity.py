@@ -31,6 +31,21 @@ def rel(a, b):
     return float(np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300))
 
 
+def spectral_divergence(uh):
+    """max |k . u_hat| / max |k| |u_hat| over the stored modes (signed wavenumbers on
+    the leading axes, i - N for i >= N/2 + 1, as leray.cpp:33-37)."""
+    dim = uh.shape[0]
+    N = uh.shape[1]
+    i = np.arange(N)
+    ks = np.where(i >= N // 2 + 1, i - N, i).astype(float)
+    kl = np.arange(uh.shape[-1], dtype=float)
+    grids = np.meshgrid(*([ks] * (dim - 1) + [kl]), indexing="ij")
+    div = sum(grids[c] * uh[c] for c in range(dim))
+    kabs = np.sqrt(sum(g * g for g in grids))
+    scale = float(np.max(kabs * np.sqrt(sum(np.abs(uh[c]) ** 2 for c in range(dim)))))
+    return float(np.max(np.abs(div))) / scale
+
+
 def random_state(dim, N, seed=3):
     print(f"[case] random fBm state dim={dim} N={N} seed={seed}", flush=True)
     return O.initialize(I.fractional_brownian(dim, N, seed=seed))
@@ -262,6 +277,8 @@ def test_trajectory_2d_dsl_100_steps(cuda):
     assert abs(sim.time() - osim.time) <= 1e-12
     assert rel(host(sim.u()), osim.u) <= 1e-11
     assert np.allclose(A.energy_spectrum(sim.u()), O.energy_spectrum(osim.u, N), rtol=1e-10, atol=1e-30)
+    # the velocity stays divergence-free to round-off
+    assert spectral_divergence(host(sim.u())) <= 1e-13
 
 
 def test_trajectory_3d_tg_20_steps(cuda):
@@ -274,6 +291,7 @@ def test_trajectory_3d_tg_20_steps(cuda):
     sim.simulate_until(1.0, max_steps=20)
     osim.simulate_until(1.0, max_steps=20)
     assert rel(host(sim.u()), osim.u) <= 1e-11
+    assert spectral_divergence(host(sim.u())) <= 1e-13
 
 
 def test_simulate_until_device_loop_matches_host_loop(cuda):
