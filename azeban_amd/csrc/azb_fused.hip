@@ -1,0 +1,77 @@
+// Fused row kernels and k-space kernel launches (the kernels are in
+// azb_kernels.hpp).  Compiled as a separate translation unit with the
+// instruction-level-parallelism scheduler, which measured faster for these
+// kernels and slower for the column passes.
+#include <hip/hip_runtime.h>
+
+#include "azb_launch.hpp"
+
+namespace azb {
+namespace {
+
+template <typename T, int L, int P>
+void launch_row4_P(const RowArgs& a, long nsamples, hipStream_t st) {
+  const long blocks = a.nlines / (2 * P) * nsamples;
+  hipLaunchKernelGGL((euler_row4_kernel<T, L, P>), dim3((unsigned)blocks), dim3(Row4Cfg<T, L, P>::NT), 0, st, a);
+}
+
+// pdiv in {1, 2}: P0 = 1536/L line pairs per workgroup, or half of that
+template <typename T, int L, int DIM>
+void launch_row_L(const RowArgs& a, long nsamples, hipStream_t st, int pdiv) {
+  constexpr int P0 = (1536 / L) < 1 ? 1 : 1536 / L;
+  constexpr int P1 = P0 / 2 >= 1 ? P0 / 2 : 1;
+  if constexpr (DIM == 3) {
+    if (pdiv == 2)
+      launch_row4_P<T, L, P1>(a, nsamples, st);
+    else
+      launch_row4_P<T, L, P0>(a, nsamples, st);
+  } else {
+    // one shape: max(768/L, 1) line pairs, 192 threads (384 at L = 1536)
+    const long blocks = a.nlines / (2 * Row2Cfg<T, L>::P) * nsamples;
+    hipLaunchKernelGGL((euler_row2_kernel<T, L>), dim3((unsigned)blocks), dim3(Row2Cfg<T, L>::NT), 0, st, a);
+    (void)pdiv;
+  }
+}
+
+template <typename T, int DIM>
+int launch_row_T(int L, const RowArgs& a, long nsamples, hipStream_t st, int pdiv) {
+  switch (L) {
+    case 96: launch_row_L<T, 96, DIM>(a, nsamples, st, pdiv); return 0;
+    case 192: launch_row_L<T, 192, DIM>(a, nsamples, st, pdiv); return 0;
+    case 384: launch_row_L<T, 384, DIM>(a, nsamples, st, pdiv); return 0;
+    case 768: launch_row_L<T, 768, DIM>(a, nsamples, st, pdiv); return 0;
+    case 1536:
+      if constexpr (DIM == 2) {
+        launch_row_L<T, 1536, 2>(a, nsamples, st, pdiv);
+        return 0;
+      }
+      return 1;
+    default: return 1;
+  }
+}
+
+}  // namespace
+
+int launch_row_fused(int tbytes, int dim, int L, const RowArgs& a, long nsamples, hipStream_t st, int pdiv) {
+  if (tbytes == 8) return dim == 3 ? launch_row_T<double, 3>(L, a, nsamples, st, pdiv)
+                                   : launch_row_T<double, 2>(L, a, nsamples, st, pdiv);
+  return dim == 3 ? launch_row_T<float, 3>(L, a, nsamples, st, pdiv)
+                  : launch_row_T<float, 2>(L, a, nsamples, st, pdiv);
+}
+
+void launch_kspace(int tbytes, int dim, const KArgs& k, unsigned blocks, unsigned samples, hipStream_t st) {
+  const dim3 grid(blocks, samples), block(256);
+  if (tbytes == 8) {
+    if (dim == 3)
+      hipLaunchKernelGGL((kspace_kernel<double, 3>), grid, block, 0, st, k);
+    else
+      hipLaunchKernelGGL((kspace_kernel<double, 2>), grid, block, 0, st, k);
+  } else {
+    if (dim == 3)
+      hipLaunchKernelGGL((kspace_kernel<float, 3>), grid, block, 0, st, k);
+    else
+      hipLaunchKernelGGL((kspace_kernel<float, 2>), grid, block, 0, st, k);
+  }
+}
+
+}  // namespace azb

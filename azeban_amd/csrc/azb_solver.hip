@@ -11,7 +11,9 @@
 #include <vector>
 
 #include "../../include/azeban_hip.h"
+#include "azb_aux_kernels.hpp"
 #include "azb_kernels.hpp"
+#include "azb_launch.hpp"
 
 using namespace azb;
 
@@ -125,50 +127,14 @@ int launch_col_full(int L, ColArgs a, long batches, hipStream_t st) {
   return AZB_OK;
 }
 
-template <typename T, int L, int P>
-void launch_row4_P(RowArgs a, long nsamples, hipStream_t st) {
-  const long blocks = a.nlines / (2 * P) * nsamples;
-  hipLaunchKernelGGL((euler_row4_kernel<T, L, P>), dim3((unsigned)blocks), dim3(Row4Cfg<T, L, P>::NT), 0, st, a);
-}
-
-// pdiv in {1, 2}: P0 = 1536/L line pairs per workgroup, or half of that
-template <typename T, int L, int DIM>
-void launch_row_L(RowArgs a, long nsamples, hipStream_t st, int pdiv) {
-  constexpr int P0 = (1536 / L) < 1 ? 1 : 1536 / L;
-  constexpr int P1 = P0 / 2 >= 1 ? P0 / 2 : 1;
-  if constexpr (DIM == 3) {
-    if (pdiv == 2)
-      launch_row4_P<T, L, P1>(a, nsamples, st);
-    else
-      launch_row4_P<T, L, P0>(a, nsamples, st);
-  } else {
-    // one shape: max(768/L, 1) line pairs, 192 threads (384 at L = 1536)
-    const long blocks = a.nlines / (2 * Row2Cfg<T, L>::P) * nsamples;
-    hipLaunchKernelGGL((euler_row2_kernel<T, L>), dim3((unsigned)blocks), dim3(Row2Cfg<T, L>::NT), 0, st, a);
-    (void)pdiv;
-  }
-}
-
 template <typename T, int DIM>
 int launch_row(int L, RowArgs a, long nsamples, hipStream_t st, int pdiv = 1) {
   int P = row_p(L);
   if (pdiv == 2 && P >= 2) P /= 2;
   if (a.nlines % (2 * P) != 0) return fail(AZB_ERR_UNSUPPORTED, "line count not a multiple of the line-pair tile");
   if (DIM == 3 && L == 1536) return fail(AZB_ERR_UNSUPPORTED, "3D row length 1536 not built");
-  switch (L) {
-#define AZB_CASE(LL) \
-  case LL: launch_row_L<T, LL, DIM>(a, nsamples, st, pdiv); break;
-    AZB_CASE(96) AZB_CASE(192) AZB_CASE(384) AZB_CASE(768)
-#undef AZB_CASE
-    case 1536:
-      if (DIM == 2) {
-        launch_row_L<T, 1536, 2>(a, nsamples, st, pdiv);
-        break;
-      }
-      return fail(AZB_ERR_UNSUPPORTED, "3D row length 1536 not built");
-    default:
-      return fail(AZB_ERR_UNSUPPORTED, "row length " + std::to_string(L) + " not built");
-  }
+  if (launch_row_fused((int)sizeof(T), DIM, L, a, nsamples, st, pdiv) != 0)
+    return fail(AZB_ERR_UNSUPPORTED, "row length " + std::to_string(L) + " not built");
   return AZB_OK;
 }
 
@@ -381,10 +347,7 @@ int enqueue_rhs_T(azb_solver* s, const double2* u_in, const double2* u0, double2
   const unsigned blocks = (unsigned)((s->nmodes + 255) / 256);
   {
     ProfScope p(s, AZB_PROF_KSPACE);
-    if (s->dim == 3)
-      hipLaunchKernelGGL((kspace_kernel<T, 3>), dim3(blocks, (unsigned)S), dim3(256), 0, st, k);
-    else
-      hipLaunchKernelGGL((kspace_kernel<T, 2>), dim3(blocks, (unsigned)S), dim3(256), 0, st, k);
+    launch_kspace((int)sizeof(T), s->dim, k, blocks, (unsigned)S, st);
   }
   HIPCHK(hipGetLastError());
   return AZB_OK;
