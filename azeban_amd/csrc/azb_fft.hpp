@@ -41,12 +41,16 @@ __device__ __forceinline__ cx<T> gload(const void* base, unsigned idx) {
   r.y = v.y;
   return r;
 }
+// Stores are non-temporal: a column pass never reads its own output, and with
+// non-temporal stores the column passes measured 5-9 % faster.  Loads stay cached
+// (neighbouring tiles share the lines that straddle tile boundaries; non-temporal
+// loads measured 7-25 % slower).
 template <typename T>
 __device__ __forceinline__ void gstore(void* base, unsigned idx, T x, T y) {
   vec2_t<T> v;
   v.x = x;
   v.y = y;
-  ((gvec2<T>*)base)[idx] = v;
+  __builtin_nontemporal_store(v, &((gvec2<T>*)base)[idx]);
 }
 
 template <typename T>

@@ -200,6 +200,19 @@ __device__ __forceinline__ double wave_max(double v) {
 }
 
 
+// 16-byte load that bypasses the caches' retention.  The k-space kernel reads
+// each input once and writes its output with non-temporal stores: measured 2-3 %
+// faster (FP32-mixed: 7 %) than cached accesses; in the row kernel non-temporal
+// loads measured slower.
+template <typename T>
+__device__ __forceinline__ cx<T> ntload(const void* p) {
+  const vec2_t<T> v = __builtin_nontemporal_load((const vec2_t<T>*)p);
+  cx<T> r;
+  r.x = v.x;
+  r.y = v.y;
+  return r;
+}
+
 // ---------------------------------------------------------------------------
 // K-space kernel with the RK stage update fused.
 // ---------------------------------------------------------------------------
@@ -252,7 +265,7 @@ __global__ __launch_bounds__(256) void kspace_kernel(KArgs a) {
   const unsigned rest = ui / (unsigned)NH;
   const long bidx = (long)rest * a.nhp + kk;
   auto B = [&](int m) -> double2 {
-    const cx<T> z = bh[(long)m * a.bstride + bidx];
+    const cx<T> z = ntload<T>(bh + (long)m * a.bstride + bidx);
     return make_double2((double)z.x, (double)z.y);
   };
   double absk2;
@@ -311,7 +324,8 @@ __global__ __launch_bounds__(256) void kspace_kernel(KArgs a) {
 #pragma unroll
   for (int c = 0; c < DIM; ++c) {
     const long o = (sample * DIM + c) * a.nmodes + idx;
-    const double2 u = a.u_in[o];
+    const cx<double> uu = ntload<double>(a.u_in + o);
+    const double2 u = make_double2(uu.x, uu.y);
     double2 d = absk2 == 0 ? make_double2(0.0, 0.0)
                            : make_double2(-L[c].x + nu * u.x, -L[c].y + nu * u.y);
     double2 r;
@@ -321,14 +335,18 @@ __global__ __launch_bounds__(256) void kspace_kernel(KArgs a) {
       r = make_double2(1.0 * u.x + dt * d.x, 1.0 * u.y + dt * d.y);
     } else {
       const double2 tmp = make_double2(1.0 * u.x + dt * d.x, 1.0 * u.y + dt * d.y);
-      const double2 w = a.u0[o];
+      const cx<double> ww = ntload<double>(a.u0 + o);
+      const double2 w = make_double2(ww.x, ww.y);
       if (a.mode == K_RK2) {
         r = make_double2(0.75 * w.x + 0.25 * tmp.x, 0.75 * w.y + 0.25 * tmp.y);
       } else {
         r = make_double2((2.0 / 3) * tmp.x + (1.0 / 3) * w.x, (2.0 / 3) * tmp.y + (1.0 / 3) * w.y);
       }
     }
-    a.out[o] = r;
+    vec2_t<double> rv;
+    rv.x = r.x;
+    rv.y = r.y;
+    __builtin_nontemporal_store(rv, (vec2_t<double>*)(a.out + o));
   }
 }
 
