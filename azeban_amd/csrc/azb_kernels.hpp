@@ -678,17 +678,24 @@ struct Row2Cfg {
   static constexpr int TSI = 2 * P, TSF = 3 * P;
   static constexpr int NB = L / 12;
   static constexpr int SOFF = 11;
-  static constexpr int XWORDS = 2 * ExLayout<T, L, TSF, true, SOFF>::elems;
+  // FP64 at L = 1536: the forward exchange goes in two halves (real, imaginary) so
+  // that three workgroups fit a CU and one launch of a single 1024^2 sample
+  // (768 workgroups) is resident at once
+  static constexpr bool FSPLIT = sizeof(T) == 8 && L == 1536;
+  static constexpr int WPE = FSPLIT ? 5 : 1;
+  static constexpr int XI = 2 * ExLayout<T, L, TSI, true, SOFF>::elems;
+  static constexpr int XF = (FSPLIT ? 1 : 2) * ExLayout<T, L, TSF, true, SOFF>::elems;
+  static constexpr int XWORDS = XI > XF ? XI : XF;
 };
 
 template <typename T, int L>
-__global__ __launch_bounds__((Row2Cfg<T, L>::NT)) void euler_row2_kernel(RowArgs a) {
+__global__ __launch_bounds__((Row2Cfg<T, L>::NT)) __attribute__((amdgpu_waves_per_eu(Row2Cfg<T, L>::WPE))) void euler_row2_kernel(RowArgs a) {
   using C = Row2Cfg<T, L>;
   using PL = Row2Plan<L>;
   constexpr int NT = C::NT, P = C::P, NB = C::NB, TSI = C::TSI, TSF = C::TSF;
   using FI = BlockFFT2<T, L, TSI, NT, +1, false, 2, MAP_STD, MAP_SEQ, true, C::SOFF>;
   constexpr int MAPL = NB <= 64 ? MAP_SEQ1 : MAP_MIRROR;
-  using FF = BlockFFT2<T, L, TSF, NT, -1, false, 3, MAP_SEQ, MAPL, true, C::SOFF>;
+  using FF = BlockFFT2<T, L, TSF, NT, -1, C::FSPLIT, 3, MAP_SEQ, MAPL, true, C::SOFF>;
   using LY = ExLayout<T, L, TSF, true, C::SOFF>;
   static_assert(FI::E == 8 && FF::E == 12, "thread layout");
   static_assert((64 % NB == 0 || NB % 64 == 0) && TSI * NB % 64 == 0, "whole waves in the radix-12 passes");
