@@ -20,7 +20,14 @@ template <typename T, int L, int DIM>
 void launch_row_L(const RowArgs& a, long nsamples, hipStream_t st, int pdiv) {
   constexpr int P0 = (1536 / L) < 1 ? 1 : 1536 / L;
   constexpr int P1 = P0 / 2 >= 1 ? P0 / 2 : 1;
-  if constexpr (DIM == 3) {
+  if constexpr (DIM == 3 && sizeof(T) == 4 && L == 768) {
+    // FP32 at L = 768: the radix-24 kernel, two line pairs per workgroup, whatever
+    // pdiv says (one transform plan per precision and size keeps results
+    // independent of the tuning)
+    const long blocks = a.nlines / (2 * Row8Cfg<T, L>::P) * nsamples;
+    hipLaunchKernelGGL((euler_row8_kernel<T, L>), dim3((unsigned)blocks), dim3(Row8Cfg<T, L>::NT), 0, st, a);
+    (void)pdiv;
+  } else if constexpr (DIM == 3) {
     if (pdiv == 2)
       launch_row4_P<T, L, P1>(a, nsamples, st);
     else

@@ -465,56 +465,61 @@ namespace azb {
 //              values a thread needs for separation come over lane permutes, and
 //              the k <= N/2 outputs leave as contiguous 16-byte stores.
 // ---------------------------------------------------------------------------
-// Packed first inverse pass of a row kernel, from HBM: radix 12 with one butterfly
-// per thread, butterfly j (< NB = L/12) of a sequence X + iY at positions j + NB r,
-// X and Y the half spectra (N/2 + 1 modes) of two real lines.  Positions up to
-// N/2 = 4 NB read k = pos (r < 4; r = 4 only for j = 0), positions from L - N/2 = 8 NB
-// read the conjugates at k = L - pos (r >= 8), the rest are the zero padding; the
-// k = 0 entry drops Im as the reference's c2r does.  Ends with the radix-12 DFT.
-template <typename T, int NB>
+// Packed first inverse pass of a row kernel, from HBM: radix R (12 or 24) with one
+// butterfly per thread, butterfly j (< NB = L/R) of a sequence X + iY at positions
+// j + NB r, X and Y the half spectra (N/2 + 1 modes) of two real lines.  Positions
+// up to N/2 = (R/3) NB read k = pos (r < R/3; r = R/3 only for j = 0), positions
+// from L - N/2 = (2R/3) NB read the conjugates at k = L - pos (r >= 2R/3), the rest
+// are the zero padding; the k = 0 entry drops Im as the reference's c2r does.  Ends
+// with the radix-R DFT.
+template <typename T, int NB, int R = 12>
 __device__ __forceinline__ void row_first_pass(cx<T>* v, const cx<T>* __restrict__ px,
                                                const cx<T>* __restrict__ py, unsigned j) {
+  static_assert(R % 3 == 0, "N/2 = L/3 must fall on a butterfly row");
+  constexpr int R3 = R / 3;
   const bool j0 = j == 0;
 #pragma unroll
-  for (int r = 0; r < 12; ++r) {
-    if (r < 4) {
+  for (int r = 0; r < R; ++r) {
+    if (r < R3) {
       const cx<T> X0 = px[j + NB * r], Y0 = py[j + NB * r];
       v[r] = mk(X0.x - Y0.y, X0.y + Y0.x);
       if (r == 0 && j0) v[r] = mk(X0.x, Y0.x);
-    } else if (r >= 8) {
-      const unsigned kk = NB * (12 - r) - j;
+    } else if (r >= 2 * R3) {
+      const unsigned kk = NB * (R - r) - j;
       const cx<T> X0 = px[kk], Y0 = py[kk];
       v[r] = mk(X0.x + Y0.y, Y0.x - X0.y);
     } else {
       v[r] = mk(T(0), T(0));
-      if (r == 4 && j0) {
-        const cx<T> X0 = px[4 * NB], Y0 = py[4 * NB];
+      if (r == R3 && j0) {
+        const cx<T> X0 = px[R3 * NB], Y0 = py[R3 * NB];
         v[r] = mk(X0.x - Y0.y, X0.y + Y0.x);
       }
     }
   }
-  DFT<12, +1, T>::run(v);
+  DFT<R, +1, T>::run(v);
 }
 
 // Separation of a packed forward transform Z = A + iB (A, B spectra of two real
-// lines) after a last pass of radix 12 with butterfly j of a sequence in lane j of
-// its wave (NB = L/12 dividing 64) or in the MAP_MIRROR layout (NB a multiple of 64).  The thread holds Z[j + NB r], r < 12; the
-// mirror of position j + NB r is (NB - j) + NB (11 - r), index 11 - r of the lane
-// holding butterfly NB - j (for j = 0: index 12 - r of this lane).  Stores
-// A[k], B[k] for the k <= N/2 = 4 NB this lane owns (r < 4, and r = 4 for j = 0) as
-// contiguous 16-byte non-temporal stores.
-template <typename T, int NB>
+// lines) after a last pass of radix R (12 or 24) with butterfly j of a sequence in
+// lane j of its wave (NB = L/R dividing 64) or in the MAP_MIRROR layout (NB a
+// multiple of 64).  The thread holds Z[j + NB r], r < R; the mirror of position
+// j + NB r is (NB - j) + NB (R - 1 - r), index R - 1 - r of the lane holding
+// butterfly NB - j (for j = 0: index R - r of this lane).  Stores A[k], B[k] for the
+// k <= N/2 = (R/3) NB this lane owns (r < R/3, and r = R/3 for j = 0) as contiguous
+// non-temporal stores.
+template <typename T, int NB, int R = 12>
 __device__ __forceinline__ void row_separate_store(const cx<T>* v, unsigned lane, unsigned j, vec2_t<T>* oa,
                                                    vec2_t<T>* ob) {
+  constexpr int R3 = R / 3;
   // NB <= 64: butterfly j in lane j of a wave per sequence; NB > 64: MAP_MIRROR
   // (mirror in lane ^ 32, the self-mirrored 0 and NB/2 in lanes 0 and 32)
   const int mlane = NB <= 64 ? (int)(lane - j + (NB - j) % NB)
                              : (int)((j == 0 || j == NB / 2) ? lane : (lane ^ 32u));
-  cx<T> f[4];
+  cx<T> f[R3];
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    f[q].x = __shfl(v[11 - q].x, mlane);
-    f[q].y = __shfl(v[11 - q].y, mlane);
+  for (int q = 0; q < R3; ++q) {
+    f[q].x = __shfl(v[R - 1 - q].x, mlane);
+    f[q].y = __shfl(v[R - 1 - q].y, mlane);
   }
   auto emit = [&](unsigned pos, const cx<T>& zk, const cx<T>& zm) {
     vec2_t<T> va, vb;
@@ -527,7 +532,7 @@ __device__ __forceinline__ void row_separate_store(const cx<T>* v, unsigned lane
   };
   const bool j0 = j == 0;
 #pragma unroll
-  for (int r = 0; r < 4; ++r) {
+  for (int r = 0; r < R3; ++r) {
     cx<T> zm = f[r];
     if (r == 0) {
       if (j0) zm = v[0];
@@ -536,7 +541,7 @@ __device__ __forceinline__ void row_separate_store(const cx<T>* v, unsigned lane
     }
     emit(j + NB * r, v[r], zm);
   }
-  if (j0) emit(4 * NB, v[4], f[3]);
+  if (j0) emit(R3 * NB, v[R3], f[R3 - 1]);
 }
 
 template <int L>
@@ -651,6 +656,130 @@ __global__ __launch_bounds__(P * L / 4) __attribute__((amdgpu_waves_per_eu(sizeo
   }
 }
 
+
+// ---------------------------------------------------------------------------
+// 3D row kernel for FP32 working precision, 24 values per thread.  Per workgroup:
+// two line pairs, six packed sequences (the row-4 kernel's three per pair), 192
+// threads.  Inverse plan 24.8.4 and forward plan 4.8.24: two exchanges per
+// transform instead of three, and the radix-8 and radix-24 passes apply fewer
+// twiddles per point than radix-4 ones.  FP32 arithmetic issues at the FP64 rate
+// here (packed FP32 is off), so the row-4 kernel is bound by instruction issue in
+// FP32 and this one trades registers (FP32 values take half of FP64's) for
+// instructions.  The radix-24 first and last passes keep butterfly j of a sequence
+// in lane j of its half-wave (NB = L/24 = 32), so the Hermitian extension and the
+// half-spectrum separation stay within a wave as in the row-4 kernel.
+// ---------------------------------------------------------------------------
+template <int L>
+struct Row8Plan;
+template <> struct Row8Plan<768> { using Inv = Plan<24, 8, 4>; using Fwd = Plan<4, 8, 24>; };
+
+template <typename T, int L>
+struct Row8Cfg {
+  static constexpr int P = 2;   // line pairs per workgroup
+  static constexpr int TS = 6;  // packed sequences
+  static constexpr int NT = 192;
+  static constexpr int E = 24;
+  static constexpr int N = 2 * L / 3;
+  static constexpr int NB = L / 24;
+  static constexpr int SOFF = 1;
+  static constexpr int XWORDS = 2 * ExLayout<T, L, TS, true, SOFF>::elems;
+};
+
+template <typename T, int L>
+__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3, 3))) void euler_row8_kernel(RowArgs a) {
+  using C = Row8Cfg<T, L>;
+  using PL = Row8Plan<L>;
+  constexpr int NT = C::NT, E = C::E, TS = C::TS, NB = C::NB, P = C::P;
+  static_assert(64 % NB == 0 && TS * L == NT * E, "thread layout");
+  using FI = BlockFFT2<T, L, TS, NT, +1, false, 6, MAP_SEQ1, MAP_SEQ, true, C::SOFF>;
+  using FF = BlockFFT2<T, L, TS, NT, -1, false, 6, MAP_SEQ, MAP_SEQ1, true, C::SOFF>;
+  static_assert(FI::E == E && FF::E == E && FI::lds_words == C::XWORDS, "exchange size");
+  static_assert(first_radix(typename PL::Inv{}) == 24 && last_radix(typename PL::Inv{}) == 4 &&
+                    first_radix(typename PL::Fwd{}) == 4 && last_radix(typename PL::Fwd{}) == 24,
+                "plans must start/end on the radices the mappings assume");
+  __shared__ T X[C::XWORDS];
+  __shared__ double red[NT / 64];
+  const long bps = a.nlines / (2 * P);  // workgroups per sample
+  const long sample = blockIdx.x / bps;
+  DevState* st = a.st + sample;
+  if (st->done) return;
+
+  const int t = threadIdx.x;
+  const long line0 = (blockIdx.x % bps) * (2 * P);
+  const cx<T>* __restrict__ in = (const cx<T>*)a.in + sample * a.in_ss;
+  cx<T>* __restrict__ out = (cx<T>*)a.out + sample * a.out_ss;
+  const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
+  const T nrm = T(a.norm);
+
+  // first inverse pass from HBM; sequence s = 3 pair + type:
+  // type 0: u1a + i u2a, type 1: u3a + i u3b, type 2: u1b + i u2b
+  cx<T> v[E];
+  {
+    unsigned tt = t;
+    asm volatile("" : "+v"(tt));
+    const unsigned s = tt / NB, j = tt % NB;
+    const unsigned pair = s / 3, type = s % 3;
+    const long la = line0 + 2 * pair, lb = la + 1;
+    const long lx = type == 0 ? la : (type == 1 ? 2 * a.nlines + la : lb);
+    const long ly = type == 0 ? a.nlines + la : (type == 1 ? 2 * a.nlines + lb : a.nlines + lb);
+    row_first_pass<T, NB, 24>(v, in + lx * a.ls, in + ly * a.ls, j);
+  }
+  FI::run_rest(v, t, X, tw, typename PL::Inv{});
+  // u[s*4 + r]: sequence s at this thread's 4 positions
+  cx<T> u[E];
+  T lm = T(0);
+#pragma unroll
+  for (int e = 0; e < E; ++e) {
+    u[e] = v[e];
+    lm = fmax(lm, fmax(fabs(v[e].x), fabs(v[e].y)));
+  }
+  double lmax = (double)lm;
+  asm volatile("" : "+v"(lmax));
+
+#pragma unroll 1
+  for (int round = 0; round < 2; ++round) {
+#pragma unroll
+    for (int pr = 0; pr < P; ++pr) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        // p0 = (u1a, u2a), p1 = (u3a, u3b), p2 = (u1b, u2b) of pair pr
+        const cx<T> p0 = u[(3 * pr) * 4 + r], p1 = u[(3 * pr + 1) * 4 + r], p2 = u[(3 * pr + 2) * 4 + r];
+        const T a1 = nrm * p0.x, a2 = nrm * p0.y, b1 = nrm * p2.x, b2 = nrm * p2.y;
+        cx<T>* w = v + (3 * pr) * 4 + r;
+        if (round == 0) {
+          w[0] = mk(a1 * a1, b1 * b1);  // B11
+          w[4] = mk(a2 * a1, b2 * b1);  // B21
+          w[8] = mk(a2 * a2, b2 * b2);  // B22
+        } else {
+          const T a3 = nrm * p1.x, b3 = nrm * p1.y;
+          w[0] = mk(a3 * a1, b3 * b1);  // B31
+          w[4] = mk(a3 * a2, b3 * b2);  // B32
+          w[8] = mk(a3 * a3, b3 * b3);  // B33
+        }
+      }
+    }
+    FF::run(v, t, X, tw, typename PL::Fwd{});
+    {
+      unsigned tt = t;
+      asm volatile("" : "+v"(tt));
+      const unsigned s = tt / NB, j = tt % NB;
+      const unsigned pair = s / 3, m = round * 3 + s % 3;
+      const long la = line0 + 2 * pair, lb = la + 1;
+      row_separate_store<T, NB, 24>(v, tt & 63, j, (vec2_t<T>*)(out + ((long)m * a.nlines + la) * a.ls),
+                                    (vec2_t<T>*)(out + ((long)m * a.nlines + lb) * a.ls));
+    }
+  }
+
+  lmax = wave_max<T>(lmax);
+  if ((t & 63) == 0) red[t >> 6] = lmax;
+  __syncthreads();
+  if (t == 0) {
+    double mx = 0.0;
+    for (int w = 0; w < NT / 64; ++w) mx = fmax(mx, red[w]);
+    atomicMax(&st->umax_bins[a.slot][blockIdx.x & (kBins - 1)],
+              (unsigned long long)__double_as_longlong(mx));
+  }
+}
 
 // ---------------------------------------------------------------------------
 // 2D row kernel (L <= 768), the 3D kernel's layout with one product round.

@@ -201,7 +201,7 @@ def embed(uh_small, n_small, N):
 
 
 @pytest.mark.parametrize("N,precision,tol", [(256, "fp64", 1e-12), (512, "fp64", 1e-12),
-                                             (512, "fp32_mixed", 3e-5)])
+                                             (256, "fp32_mixed", 2e-5), (512, "fp32_mixed", 3e-5)])
 def test_dudt_3d_full_size_band_limited(cuda, N, precision, tol):
     """Full-size 3D RHS (every row line of the padded grid carries data) against the
     oracle on a 32^3 grid: a field with |k_i| <= 3 has products with |k_i| <= 6, which
@@ -210,7 +210,8 @@ def test_dudt_3d_full_size_band_limited(cuda, N, precision, tol):
     FP32 bar: the same RHS through float32 transforms and products on the CPU (scipy's
     pocketfft) has relative L2 error 0.6e-6 / 1.2e-6 / 2.6e-6 at N = 32 / 64 / 128, growing
     in proportion to N (the rounding floor spreads over all N^3 modes while the RHS
-    sits on a few hundred), so about 1e-5 is the float32 floor at N = 512."""
+    sits on a few hundred), so about 1e-5 is the float32 floor at N = 512.  FP32 at
+    N = 512 runs the radix-24 row kernel, at N = 256 the radix-12 one."""
     print(f"[case] 3D band-limited dudt N={N} {precision}", flush=True)
     n0 = 32
     uh0 = band_limited_state(3, n0, 3, seed=21)
