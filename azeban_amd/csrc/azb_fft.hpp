@@ -473,10 +473,14 @@ namespace azb {
 //             its mirror NB - j sit in lanes l and l ^ 32 of one wave: lanes 0..31
 //             of wave w hold 32 w + l, lanes 32..63 the mirrors (wave 0 pairs the
 //             two self-mirrored butterflies 0 and NB/2 in lanes 0 and 32)
+//   MAP_WAVE: wave w owns sequence w (NT = 64 TS and L = 64 E), lane l its
+//             butterflies l + 64 b; an exchange between two passes that keep every
+//             sequence inside its own wave (MAP_WAVE, or MAP_SEQ1 with L/R = 64)
+//             moves data only within a wave and needs no workgroup barrier
 // Any mapping can be used by any pass: the LDS exchange writes with the mapping of
 // the pass that produced the data and reads with the mapping of the next pass.
 // ---------------------------------------------------------------------------
-enum { MAP_STD = 0, MAP_SEQ = 1, MAP_SEQ1 = 3, MAP_MIRROR = 4 };
+enum { MAP_STD = 0, MAP_SEQ = 1, MAP_SEQ1 = 3, MAP_MIRROR = 4, MAP_WAVE = 5 };
 
 // Index arithmetic is unsigned throughout: signed division and modulo by a power of
 // two cost a sign fix-up each, and these run once per element of every exchange.
@@ -491,6 +495,10 @@ __device__ __forceinline__ void bmap(unsigned t, unsigned b, unsigned& s, unsign
       s = g % TS;
       j = g / TS;
     }
+  } else if constexpr (MAP == MAP_WAVE) {
+    static_assert(NT % 64 == 0, "whole waves");
+    s = t / 64u;
+    j = t % 64u + 64u * b;
   } else if constexpr (MAP == MAP_SEQ || MAP == MAP_SEQ1) {
     constexpr unsigned GG = MAP == MAP_SEQ1 ? 1 : G;
     const unsigned q = b / GG, gg = b % GG;
@@ -536,18 +544,38 @@ struct ExLayout {
   static constexpr unsigned delta(int D) { return SWZ ? D + D / PADE : D * TS; }
 };
 
+// MIDW: bit P set = middle pass P uses MAP_WAVE instead of MAP_STD.
 template <typename T, int L, int TS, int NT, int DIR, bool SPLIT, int G, int MAPF, int MAPL, bool SWZ = false,
-          int SOFF = 1>
+          int SOFF = 1, unsigned MIDW = 0>
 struct BlockFFT2 {
   static constexpr int E = TS * L / NT;
   static_assert(TS * L % NT == 0, "TS*L must be a multiple of NT");
+  static_assert(MIDW == 0 || (NT == 64 * TS && L == 64 * E), "MAP_WAVE needs one wave per sequence");
   using LY = ExLayout<T, L, TS, SWZ, SOFF>;
   // LDS words (of T) the exchange needs
   static constexpr int lds_words = SPLIT ? LY::elems : 2 * LY::elems;
 
   template <int P, int NP>
   static constexpr int map_of() {
-    return P == 0 ? MAPF : (P == NP - 1 ? MAPL : MAP_STD);
+    return P == 0 ? MAPF : (P == NP - 1 ? MAPL : (((MIDW >> P) & 1u) ? MAP_WAVE : MAP_STD));
+  }
+  // every sequence stays inside its own wave (sequence s = wave s) under this mapping
+  template <int MAP, int R>
+  static constexpr bool wave_local() {
+    return NT == 64 * TS && (MAP == MAP_WAVE || (MAP == MAP_SEQ1 && L / R == 64));
+  }
+  // Between the write and the read phases of an exchange, and before the writes
+  // (earlier reads of the same words): a workgroup barrier, or, when both mappings
+  // keep each sequence in its own wave, only this wave's LDS operations completing
+  // (a wave's LDS instructions execute in order; the wait also keeps the compiler
+  // from moving LDS accesses across it).  A wave-local exchange follows a pass whose
+  // mapping is wave-local, so the previous exchange's reads were this wave's own.
+  template <bool LOCAL>
+  __device__ __forceinline__ static void exchange_sync() {
+    if constexpr (LOCAL)
+      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    else
+      __syncthreads();
   }
 
   // twiddle table pointer in the global address space: global (not flat) loads
@@ -607,6 +635,7 @@ struct BlockFFT2 {
   template <int MAPW, int MAPR, int Ns, int R, int Rn>
   __device__ __forceinline__ static void exchange(cx<T>* v, int t_in, T* lds) {
     constexpr int comps = SPLIT ? 2 : 1;
+    constexpr bool local = wave_local<MAPW, R>() && wave_local<MAPR, Rn>();
     // opaque copy of the thread index: LDS addresses are recomputed here instead of
     // being hoisted (they are loop-invariant across transforms) and held live
     unsigned t = t_in;
@@ -614,7 +643,7 @@ struct BlockFFT2 {
     __builtin_assume(t < (unsigned)NT);
 #pragma unroll
     for (int c = 0; c < comps; ++c) {
-      __syncthreads();
+      exchange_sync<local>();
 #pragma unroll
       for (int b = 0; b < E / R; ++b) {
         unsigned s, j;
@@ -633,7 +662,7 @@ struct BlockFFT2 {
           }
         }
       }
-      __syncthreads();
+      exchange_sync<local>();
 #pragma unroll
       for (int b = 0; b < E / Rn; ++b) {
         unsigned s, j;

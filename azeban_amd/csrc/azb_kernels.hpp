@@ -560,6 +560,12 @@ struct Row4Cfg {
   static constexpr int NB = L / 12;  // butterflies per sequence in the radix-12 passes
   static constexpr int SOFF = 11;
   static constexpr int XWORDS = 2 * ExLayout<T, L, TS, true, SOFF>::elems;
+  // at L = 768 (one wave per sequence in the radix-12 passes, L = 64 E) the inverse
+  // runs its two middle passes and the forward its last radix-4 pass with each
+  // sequence inside its own wave: three of the nine exchanges per line pair (and
+  // one of each forward round) then need no workgroup barrier
+  static constexpr unsigned MIDW_INV = L == 64 * E ? 0b0110u : 0u;
+  static constexpr unsigned MIDW_FWD = L == 64 * E ? 0b0100u : 0u;
 };
 
 template <typename T, int L, int P>
@@ -568,8 +574,8 @@ __global__ __launch_bounds__(P * L / 4) __attribute__((amdgpu_waves_per_eu(sizeo
   using PL = Row4Plan<L>;
   constexpr int NT = C::NT, E = C::E, N = C::N, TS = C::TS, NB = C::NB;
   static_assert(64 % NB == 0 && NT % 64 == 0, "a wave must hold whole sequences in the radix-12 passes");
-  using FI = BlockFFT2<T, L, TS, NT, +1, false, 3, MAP_SEQ1, MAP_SEQ, true, C::SOFF>;
-  using FF = BlockFFT2<T, L, TS, NT, -1, false, 3, MAP_SEQ, MAP_SEQ1, true, C::SOFF>;
+  using FI = BlockFFT2<T, L, TS, NT, +1, false, 3, MAP_SEQ1, MAP_SEQ, true, C::SOFF, C::MIDW_INV>;
+  using FF = BlockFFT2<T, L, TS, NT, -1, false, 3, MAP_SEQ, MAP_SEQ1, true, C::SOFF, C::MIDW_FWD>;
   static_assert(FI::E == E && FF::E == E && FI::lds_words == C::XWORDS, "exchange size");
   static_assert(first_radix(typename PL::Inv{}) == 12 && last_radix(typename PL::Inv{}) == 4 &&
                     first_radix(typename PL::Fwd{}) == 4 && last_radix(typename PL::Fwd{}) == 12,
