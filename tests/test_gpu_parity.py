@@ -227,6 +227,27 @@ def test_dudt_3d_full_size_band_limited(cuda, N, precision, tol):
     assert err <= tol
 
 
+def test_row_kernel_tiles_identical_at_full_size(cuda):
+    """At M = 768 the FP64 row kernel runs with one line pair per workgroup (the
+    default) or two (pdiv = 1, a shape the autotuner may pick); both use the same
+    plans, wave-private exchanges and arithmetic per line pair, so the RHS must be
+    bit-identical."""
+    N = 512
+    print(f"[case] row kernel tiles N={N}", flush=True)
+    g = torch.Generator(device="cuda").manual_seed(22)
+    u = torch.randn((3, N, N, N // 2 + 1), dtype=torch.complex128, device="cuda", generator=g)
+    eq = A.IncompressibleEuler(A.Grid(3, N), A.SmoothCutoff1D(0.05 / N, 1.0, 2 * math.pi * N / 4))
+    outs = []
+    for pdiv in (2, 1):
+        eq.set_tuning([1] * 6, pdiv)
+        assert eq.get_tuning()[1] == pdiv
+        out = torch.empty_like(u)
+        eq.dudt(out, u)
+        outs.append(out)
+    torch.cuda.synchronize()
+    assert torch.equal(outs[0], outs[1])
+
+
 def test_dudt_in_place(cuda):
     """dudt(x, x) aliasing contract (test/azeban/equations/incompressible_euler.cpp:194)."""
     uh = random_state(2, 128)
