@@ -229,23 +229,25 @@ def test_dudt_3d_full_size_band_limited(cuda, N, precision, tol):
 
 def test_row_kernel_tiles_identical_at_full_size(cuda):
     """At M = 768 the FP64 row kernel runs with one line pair per workgroup (the
-    default) or two (pdiv = 1, a shape the autotuner may pick); both use the same
-    plans, wave-private exchanges and arithmetic per line pair, so the RHS must be
-    bit-identical."""
+    default) or two (pdiv = 1), and the column passes with tiles of 8, 4 or 2
+    columns (tsdiv 1, 2, 4): shapes the autotuner may pick.  All use the same plans
+    and arithmetic per line or column, so the RHS must be bit-identical."""
     N = 512
     print(f"[case] row kernel tiles N={N}", flush=True)
     g = torch.Generator(device="cuda").manual_seed(22)
     u = torch.randn((3, N, N, N // 2 + 1), dtype=torch.complex128, device="cuda", generator=g)
     eq = A.IncompressibleEuler(A.Grid(3, N), A.SmoothCutoff1D(0.05 / N, 1.0, 2 * math.pi * N / 4))
-    outs = []
-    for pdiv in (2, 1):
-        eq.set_tuning([1] * 6, pdiv)
-        assert eq.get_tuning()[1] == pdiv
+    ref = None
+    for tsdiv, pdiv in (([1] * 6, 2), ([1] * 6, 1), ([2] * 6, 2), ([4] * 6, 2)):
+        eq.set_tuning(tsdiv, pdiv)
+        assert eq.get_tuning() == (tsdiv, pdiv)
         out = torch.empty_like(u)
         eq.dudt(out, u)
-        outs.append(out)
-    torch.cuda.synchronize()
-    assert torch.equal(outs[0], outs[1])
+        if ref is None:
+            ref = out
+        else:
+            torch.cuda.synchronize()
+            assert torch.equal(out, ref), (tsdiv, pdiv)
 
 
 def test_dudt_in_place(cuda):
