@@ -46,7 +46,12 @@ int launch_row_T(int L, const RowArgs& a, long nsamples, hipStream_t st, int pdi
     case 96: launch_row_L<T, 96, DIM>(a, nsamples, st, pdiv); return 0;
     case 192: launch_row_L<T, 192, DIM>(a, nsamples, st, pdiv); return 0;
     case 384: launch_row_L<T, 384, DIM>(a, nsamples, st, pdiv); return 0;
-    case 768: launch_row_L<T, 768, DIM>(a, nsamples, st, pdiv); return 0;
+    case 768:
+      if constexpr (DIM == 3 && sizeof(T) == 4) {
+        if (a.nlines % (2 * Row8Cfg<T, 768>::P) != 0) return 2;
+      }
+      launch_row_L<T, 768, DIM>(a, nsamples, st, pdiv);
+      return 0;
     case 1536:
       if constexpr (DIM == 2) {
         launch_row_L<T, 1536, 2>(a, nsamples, st, pdiv);

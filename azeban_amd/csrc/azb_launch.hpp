@@ -11,8 +11,10 @@ namespace azb {
 
 // Fused row kernel (inverse, max|u|, products, forward) over the lines of a:
 // tbytes = sizeof(working real), dim 2 or 3, L the padded line length, pdiv the
-// line-pair divisor of the 3D kernel.  Returns 0, or 1 when no kernel is built
-// for (dim, L); the caller has checked the line count against the tile.
+// line-pair divisor of the 3D kernel.  Returns 0; 1 when no kernel is built for
+// (dim, L); 2 when the line count is not a multiple of the kernel's line-pair
+// tile (the caller checks the tile that pdiv selects; the FP32 kernel at
+// L = 768 always takes two pairs).
 int launch_row_fused(int tbytes, int dim, int L, const RowArgs& a, long nsamples, hipStream_t st, int pdiv);
 
 // k-space kernel with the RK stage update over blocks x samples workgroups of 256.

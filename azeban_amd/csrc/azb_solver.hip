@@ -133,8 +133,9 @@ int launch_row(int L, RowArgs a, long nsamples, hipStream_t st, int pdiv = 1) {
   if (pdiv == 2 && P >= 2) P /= 2;
   if (a.nlines % (2 * P) != 0) return fail(AZB_ERR_UNSUPPORTED, "line count not a multiple of the line-pair tile");
   if (DIM == 3 && L == 1536) return fail(AZB_ERR_UNSUPPORTED, "3D row length 1536 not built");
-  if (launch_row_fused((int)sizeof(T), DIM, L, a, nsamples, st, pdiv) != 0)
-    return fail(AZB_ERR_UNSUPPORTED, "row length " + std::to_string(L) + " not built");
+  const int r = launch_row_fused((int)sizeof(T), DIM, L, a, nsamples, st, pdiv);
+  if (r == 2) return fail(AZB_ERR_UNSUPPORTED, "line count not a multiple of the line-pair tile");
+  if (r != 0) return fail(AZB_ERR_UNSUPPORTED, "row length " + std::to_string(L) + " not built");
   return AZB_OK;
 }
 
