@@ -305,6 +305,29 @@ def test_trajectory_2d_dsl_100_steps(cuda):
     assert spectral_divergence(host(sim.u())) <= 1e-13
 
 
+def test_taylor_green_2d_128_to_t1(cuda):
+    """The CPU-runnable reference case: 2D Taylor-Green, N = 128, SSP-RK3 with
+    C = 0.2 and SmoothCutoff(0.05/N, 1, 2 pi N/4) to t = 1 (about 285 steps), device
+    loop against the FP64 oracle's host loop: same step count and time, relative L2
+    <= 1e-11, the same energy spectrum, and a divergence-free velocity."""
+    N = 128
+    print("[case] TG2D N=128 to t=1", flush=True)
+    uh = O.initialize(I.taylor_green_2d(N))
+    g, eq, oeq = make_pair(2, N)
+    sim = A.Simulation(g, 0.2, A.SSP_RK3(g.shape_fourier(2), eq), dev(uh))
+    osim = O.Simulation(oeq, uh, 0.2)
+    steps = sim.simulate_until(1.0)
+    osim.simulate_until(1.0)
+    assert steps == len(osim.dts) and steps > 250
+    assert sim.time() == osim.time == 1.0
+    got = host(sim.u())
+    assert rel(got, osim.u) <= 1e-11
+    # all energy sits in the shell K = 1; the other shells hold round-off only
+    e_ref = O.energy_spectrum(osim.u, N)
+    assert np.allclose(A.energy_spectrum(sim.u()), e_ref, rtol=1e-10, atol=1e-12 * e_ref.sum())
+    assert spectral_divergence(got) <= 1e-13
+
+
 def test_trajectory_3d_tg_20_steps(cuda):
     N = 64
     print("[case] TG3D N=64 20 steps", flush=True)
