@@ -396,6 +396,30 @@ def test_fp32_mixed_error(cuda, dim, N, steps):
     assert err <= 1e-5
 
 
+def test_fp32_mixed_taylor_green_512_100_steps(cuda):
+    """The benchmark's FP32-mixed form at its own size: 3D Taylor-Green 512^3,
+    100 SSP-RK3 steps, against the FP64 path (which the smaller-size tests tie to
+    the oracle at 1e-11): relative L2 <= 1e-5, and the same step count."""
+    N = 512
+    print(f"[case] fp32_mixed TG3D N={N} 100 steps", flush=True)
+    u0 = A.initialize(I.taylor_green_3d(N))
+    res = {}
+    for prec in ("fp64", "fp32_mixed"):
+        g, eq, _ = make_pair(3, N, prec)
+        sim = A.Simulation(g, 0.2, A.SSP_RK3(g.shape_fourier(3), eq), u0)
+        n = sim.simulate_until(10.0, max_steps=100, chunk=20)
+        assert n == 100
+        res[prec] = (sim.u().clone(), sim.time())
+        del sim, eq
+        torch.cuda.empty_cache()
+    a, ta = res["fp64"]
+    b, tb = res["fp32_mixed"]
+    err = float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(a))
+    print(f"fp32_mixed rel L2 vs FP64 after 100 steps at 512^3: {err:.3e}; t {ta} vs {tb}")
+    assert err <= 1e-5
+    assert abs(ta - tb) <= 1e-6 * ta
+
+
 def test_copy_bandwidth_probe(cuda):
     """The streaming-copy reference used for the measured roofline returns a sane rate."""
     import ctypes as C
