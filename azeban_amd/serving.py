@@ -49,6 +49,9 @@ class _Active:
     admitted: float
     steps: int = 0
     probing: bool = False
+    # pinned host copy of a host-side initial state: kept alive until the request
+    # finishes, since the asynchronous upload on the request's stream reads it
+    host_src: torch.Tensor | None = None
 
 
 @dataclass
@@ -101,16 +104,17 @@ class Server:
                 check(lib.azb_create(r.dim, r.N, 0, r.dim, PRECISIONS[r.precision], visc, st, C.byref(h)))
                 u = torch.empty(r.u_hat.shape, dtype=torch.complex128, device="cuda")
                 src = r.u_hat.detach().to(torch.complex128)
+                host_src = None
                 if src.is_cuda:
                     torch.cuda.synchronize()
                     u.copy_(src)
                     torch.cuda.synchronize()
                 else:
-                    src = src.contiguous().pin_memory()
-                    check(lib.azb_memcpy_h2d_async(C.c_void_p(u.data_ptr()), C.c_void_p(src.data_ptr()),
+                    host_src = src.contiguous().pin_memory()
+                    check(lib.azb_memcpy_h2d_async(C.c_void_p(u.data_ptr()), C.c_void_p(host_src.data_ptr()),
                                                    u.numel() * 16, st))
                 check(lib.azb_loop_begin(h, r.t_end, self.C, None))
-                a = _Active(r, h, st, u, need, now())
+                a = _Active(r, h, st, u, need, now(), host_src=host_src)
                 check(lib.azb_enqueue_steps(h, C.c_void_p(u.data_ptr()), self.chunk_steps))
                 check(lib.azb_loop_probe(h))
                 a.probing = True

@@ -48,3 +48,20 @@ def test_budget_serialises_admission(cuda):
     stats = srv.run(rs)
     assert stats["requests"] == 3
     assert stats["peak_active"] == 1
+
+
+def test_host_side_requests_upload_asynchronously(cuda):
+    """Requests whose state is in host memory are uploaded on their own stream from a
+    pinned copy the server keeps until they finish; results equal those of the same
+    requests given on the device."""
+    print("[case] serving host inputs", flush=True)
+    rs = reqs()
+    host_rs = [Request(r.rid, r.dim, r.N, r.u_hat.cpu(), r.t_end, r.arrival) for r in rs]
+    a, b = Server(chunk_steps=3, max_active=8), Server(chunk_steps=3, max_active=8)
+    a.run(rs)
+    b.run(host_rs)
+    torch.cuda.synchronize()
+    ra = {r.rid: r for r in a.results}
+    for res in b.results:
+        assert res.steps == ra[res.rid].steps
+        assert torch.equal(res.u_hat, ra[res.rid].u_hat), res.rid
