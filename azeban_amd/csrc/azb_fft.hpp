@@ -531,9 +531,11 @@ struct PlanAt<R0, Rs...> {
 // either index stay spread over the banks (SOFF = 1 suits six interleaved
 // sequences; three interleaved sequences with consecutive positions want the
 // three runs of five or six slots apart, SOFF = 11).
-template <typename T, int L, int TS, bool SWZ, int SOFF = 1>
+template <typename T, int L, int TS, bool SWZ, int SOFF = 1, int PADE_ = 0>
 struct ExLayout {
-  static constexpr int PADE = 256 / (2 * (int)sizeof(T));
+  // elements per 256-byte bank row (a pad slot follows each); a split exchange
+  // (real and imaginary parts in turn) stores one T per slot
+  static constexpr int PADE = PADE_ ? PADE_ : 256 / (2 * (int)sizeof(T));
   static constexpr int LP = ((L + L / PADE + PADE - 1) / PADE) * PADE + SOFF;
   static constexpr int elems = SWZ ? TS * LP : TS * L;
   __device__ __forceinline__ static unsigned at(unsigned s, unsigned pos) {
@@ -544,14 +546,15 @@ struct ExLayout {
   static constexpr unsigned delta(int D) { return SWZ ? D + D / PADE : D * TS; }
 };
 
-// MIDW: bit P set = middle pass P uses MAP_WAVE instead of MAP_STD.
+// MIDW: bit P set = middle pass P uses MAP_WAVE instead of MAP_STD.  PADE: pad
+// period of the swizzled layout in elements (0: one 16-byte complex per slot).
 template <typename T, int L, int TS, int NT, int DIR, bool SPLIT, int G, int MAPF, int MAPL, bool SWZ = false,
-          int SOFF = 1, unsigned MIDW = 0>
+          int SOFF = 1, unsigned MIDW = 0, int PADE = 0>
 struct BlockFFT2 {
   static constexpr int E = TS * L / NT;
   static_assert(TS * L % NT == 0, "TS*L must be a multiple of NT");
   static_assert(MIDW == 0 || (NT == 64 * TS && L == 64 * E), "MAP_WAVE needs one wave per sequence");
-  using LY = ExLayout<T, L, TS, SWZ, SOFF>;
+  using LY = ExLayout<T, L, TS, SWZ, SOFF, PADE>;
   // LDS words (of T) the exchange needs
   static constexpr int lds_words = SPLIT ? LY::elems : 2 * LY::elems;
 

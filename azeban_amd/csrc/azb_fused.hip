@@ -27,6 +27,12 @@ void launch_row_L(const RowArgs& a, long nsamples, hipStream_t st, int pdiv) {
     const long blocks = a.nlines / (2 * Row8Cfg<T, L>::P) * nsamples;
     hipLaunchKernelGGL((euler_row8_kernel<T, L>), dim3((unsigned)blocks), dim3(Row8Cfg<T, L>::NT), 0, st, a);
     (void)pdiv;
+  } else if constexpr (DIM == 3 && sizeof(T) == 8 && L == 768) {
+    // FP64 at L = 768: one line pair per workgroup with a single six-sequence
+    // forward round, whatever pdiv says (one plan per precision and size)
+    const long blocks = a.nlines / 2 * nsamples;
+    hipLaunchKernelGGL((euler_row6_kernel<T>), dim3((unsigned)blocks), dim3(Row6Cfg::NT), 0, st, a);
+    (void)pdiv;
   } else if constexpr (DIM == 3) {
     if (pdiv == 2)
       launch_row4_P<T, L, P1>(a, nsamples, st);
@@ -50,6 +56,7 @@ int launch_row_T(int L, const RowArgs& a, long nsamples, hipStream_t st, int pdi
       if constexpr (DIM == 3 && sizeof(T) == 4) {
         if (a.nlines % (2 * Row8Cfg<T, 768>::P) != 0) return 2;
       }
+      if (a.nlines % 2 != 0) return 2;
       launch_row_L<T, 768, DIM>(a, nsamples, st, pdiv);
       return 0;
     case 1536:

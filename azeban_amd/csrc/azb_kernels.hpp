@@ -546,7 +546,6 @@ __device__ __forceinline__ void row_separate_store(const cx<T>* v, unsigned lane
 
 template <int L>
 struct Row4Plan;
-template <> struct Row4Plan<768> { using Inv = Plan<12, 4, 4, 4>; using Fwd = Plan<4, 4, 4, 12>; };
 template <> struct Row4Plan<384> { using Inv = Plan<12, 2, 4, 4>; using Fwd = Plan<4, 4, 2, 12>; };
 template <> struct Row4Plan<192> { using Inv = Plan<12, 4, 4>; using Fwd = Plan<4, 4, 12>; };
 template <> struct Row4Plan<96> { using Inv = Plan<12, 2, 4>; using Fwd = Plan<4, 2, 12>; };
@@ -560,12 +559,6 @@ struct Row4Cfg {
   static constexpr int NB = L / 12;  // butterflies per sequence in the radix-12 passes
   static constexpr int SOFF = 11;
   static constexpr int XWORDS = 2 * ExLayout<T, L, TS, true, SOFF>::elems;
-  // at L = 768 (one wave per sequence in the radix-12 passes, L = 64 E) the inverse
-  // runs its two middle passes and the forward its last radix-4 pass with each
-  // sequence inside its own wave: three of the nine exchanges per line pair (and
-  // one of each forward round) then need no workgroup barrier
-  static constexpr unsigned MIDW_INV = L == 64 * E ? 0b0110u : 0u;
-  static constexpr unsigned MIDW_FWD = L == 64 * E ? 0b0100u : 0u;
 };
 
 template <typename T, int L, int P>
@@ -574,8 +567,8 @@ __global__ __launch_bounds__(P * L / 4) __attribute__((amdgpu_waves_per_eu(sizeo
   using PL = Row4Plan<L>;
   constexpr int NT = C::NT, E = C::E, N = C::N, TS = C::TS, NB = C::NB;
   static_assert(64 % NB == 0 && NT % 64 == 0, "a wave must hold whole sequences in the radix-12 passes");
-  using FI = BlockFFT2<T, L, TS, NT, +1, false, 3, MAP_SEQ1, MAP_SEQ, true, C::SOFF, C::MIDW_INV>;
-  using FF = BlockFFT2<T, L, TS, NT, -1, false, 3, MAP_SEQ, MAP_SEQ1, true, C::SOFF, C::MIDW_FWD>;
+  using FI = BlockFFT2<T, L, TS, NT, +1, false, 3, MAP_SEQ1, MAP_SEQ, true, C::SOFF>;
+  using FF = BlockFFT2<T, L, TS, NT, -1, false, 3, MAP_SEQ, MAP_SEQ1, true, C::SOFF>;
   static_assert(FI::E == E && FF::E == E && FI::lds_words == C::XWORDS, "exchange size");
   static_assert(first_radix(typename PL::Inv{}) == 12 && last_radix(typename PL::Inv{}) == 4 &&
                     first_radix(typename PL::Fwd{}) == 4 && last_radix(typename PL::Fwd{}) == 12,
@@ -784,6 +777,107 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
   if (t == 0) {
     double mx = 0.0;
     for (int w = 0; w < NT / 64; ++w) mx = fmax(mx, red[w]);
+    atomicMax(&st->umax_bins[a.slot][blockIdx.x & (kBins - 1)],
+              (unsigned long long)__double_as_longlong(mx));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3D FP64 row kernel at L = 768 with one forward round.  The inverse is the row-4
+// kernel's (three packed sequences of one line pair, 12 values per thread, plan
+// 12.4.4.4).  All six products B_m (packed line a + i line b) are formed at once
+// from the inverse output, so the inverse values need not stay in registers for a
+// second round, and one forward transform of six sequences (24 values per thread,
+// plan 4.8.24, two exchanges) replaces the row-4 kernel's two rounds of three
+// (three exchanges each).  The forward exchanges go in two halves (real,
+// imaginary: 8-byte slots, pad period 32) so the exchange buffer keeps the size
+// of the inverse's.  The radix-24 last pass keeps butterfly j of sequence m in
+// lane j of a half-wave, so the separation of each packed spectrum stays within a
+// wave as in the other row kernels.
+// ---------------------------------------------------------------------------
+struct Row6Cfg {
+  static constexpr int L = 768;
+  static constexpr int NT = 192;
+  static constexpr int EI = 12, EF = 24;
+  static constexpr int NBI = L / 12, NBF = L / 24;
+  static constexpr int SOFFI = 11, SOFFF = 5;
+  static constexpr int XI = 2 * ExLayout<double, L, 3, true, SOFFI>::elems;
+  static constexpr int XF = ExLayout<double, L, 6, true, SOFFF, 32>::elems;
+  static constexpr int XWORDS = XI > XF ? XI : XF;
+};
+
+template <typename T>
+__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3, 3))) void euler_row6_kernel(RowArgs a) {
+  static_assert(sizeof(T) == 8, "FP64 kernel");
+  using C = Row6Cfg;
+  constexpr int L = C::L, NT = C::NT, EI = C::EI, EF = C::EF, NBI = C::NBI, NBF = C::NBF;
+  // inverse middle passes with each sequence in its own wave (MAP_WAVE): two of
+  // its three exchanges need no workgroup barrier
+  using FI = BlockFFT2<T, L, 3, NT, +1, false, 3, MAP_SEQ1, MAP_SEQ, true, C::SOFFI, 0b0110u>;
+  using FF = BlockFFT2<T, L, 6, NT, -1, true, 6, MAP_SEQ, MAP_SEQ1, true, C::SOFFF, 0u, 32>;
+  static_assert(FI::E == EI && FF::E == EF, "thread layout");
+  static_assert(FI::lds_words <= C::XWORDS && FF::lds_words <= C::XWORDS, "exchange size");
+  __shared__ T X[C::XWORDS];
+  __shared__ double red[NT / 64];
+  const long bps = a.nlines / 2;  // workgroups per sample
+  const long sample = blockIdx.x / bps;
+  DevState* st = a.st + sample;
+  if (st->done) return;
+
+  const int t = threadIdx.x;
+  const long la = (blockIdx.x % bps) * 2, lb = la + 1;
+  const cx<T>* __restrict__ in = (const cx<T>*)a.in + sample * a.in_ss;
+  cx<T>* __restrict__ out = (cx<T>*)a.out + sample * a.out_ss;
+  const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
+  const T nrm = T(a.norm);
+
+  // inverse: sequence s = type: 0: u1a + i u2a, 1: u3a + i u3b, 2: u1b + i u2b
+  cx<T> u[EI];
+  {
+    unsigned tt = t;
+    asm volatile("" : "+v"(tt));
+    const unsigned s = tt / NBI, j = tt % NBI;
+    const long lx = s == 0 ? la : (s == 1 ? 2 * a.nlines + la : lb);
+    const long ly = s == 0 ? a.nlines + la : (s == 1 ? 2 * a.nlines + lb : a.nlines + lb);
+    row_first_pass<T, NBI>(u, in + lx * a.ls, in + ly * a.ls, j);
+  }
+  FI::run_rest(u, t, X, tw, Plan<12, 4, 4, 4>{});
+  T lm = T(0);
+#pragma unroll
+  for (int e = 0; e < EI; ++e) lm = fmax(lm, fmax(fabs(u[e].x), fabs(u[e].y)));
+  double lmax = (double)lm;
+  asm volatile("" : "+v"(lmax));
+
+  // products at this thread's 4 positions: w[m*4 + r] = B_m (line a) + i B_m (line b),
+  // m = 0..5: B11, B21, B22, B31, B32, B33
+  cx<T> w[EF];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const cx<T> p0 = u[r], p1 = u[4 + r], p2 = u[8 + r];
+    const T a1 = nrm * p0.x, a2 = nrm * p0.y, a3 = nrm * p1.x;
+    const T b1 = nrm * p2.x, b2 = nrm * p2.y, b3 = nrm * p1.y;
+    w[0 * 4 + r] = mk(a1 * a1, b1 * b1);
+    w[1 * 4 + r] = mk(a2 * a1, b2 * b1);
+    w[2 * 4 + r] = mk(a2 * a2, b2 * b2);
+    w[3 * 4 + r] = mk(a3 * a1, b3 * b1);
+    w[4 * 4 + r] = mk(a3 * a2, b3 * b2);
+    w[5 * 4 + r] = mk(a3 * a3, b3 * b3);
+  }
+  FF::run(w, t, X, tw, Plan<4, 8, 24>{});
+  {
+    unsigned tt = t;
+    asm volatile("" : "+v"(tt));
+    const unsigned m = tt / NBF, j = tt % NBF;
+    row_separate_store<T, NBF, 24>(w, tt & 63, j, (vec2_t<T>*)(out + ((long)m * a.nlines + la) * a.ls),
+                                   (vec2_t<T>*)(out + ((long)m * a.nlines + lb) * a.ls));
+  }
+
+  lmax = wave_max<T>(lmax);
+  if ((t & 63) == 0) red[t >> 6] = lmax;
+  __syncthreads();
+  if (t == 0) {
+    double mx = 0.0;
+    for (int q = 0; q < NT / 64; ++q) mx = fmax(mx, red[q]);
     atomicMax(&st->umax_bins[a.slot][blockIdx.x & (kBins - 1)],
               (unsigned long long)__double_as_longlong(mx));
   }

@@ -48,7 +48,7 @@ def dsl_device(N: int, seed: int = 1):
 
 
 # kernel of each family at the 512^3 FP64 workload, as named in the counter summary
-_FAMILY_KERNEL = {"row_fused": "azb::euler_row4_kernel<double, 768,", "kspace_rk": "azb::kspace_kernel<double, 3>"}
+_FAMILY_KERNEL = {"row_fused": "azb::euler_row6_kernel<double>", "kspace_rk": "azb::kspace_kernel<double, 3>"}
 COUNTERS_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "counters_tg3d_512.json")
 
 
