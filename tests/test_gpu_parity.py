@@ -341,6 +341,21 @@ def test_trajectory_3d_tg_20_steps(cuda):
     assert spectral_divergence(host(sim.u())) <= 1e-13
 
 
+@pytest.mark.parametrize("dim,N,steps,ic", [(3, 128, 4, "tg"), (2, 1024, 5, "dsl")])
+def test_trajectory_large_grids(cuda, dim, N, steps, ic):
+    """A few SSP-RK3 steps on the larger grids (3D 128^3: M = 192 kernels; 2D 1024^2:
+    M = 1536 kernels with the split forward exchange) against the FP64 oracle."""
+    print(f"[case] trajectory dim={dim} N={N} {ic} {steps} steps", flush=True)
+    uh = O.initialize(I.taylor_green_3d(N) if ic == "tg" else I.double_shear_layer(N, seed=3))
+    g, eq, oeq = make_pair(dim, N)
+    sim = A.Simulation(g, 0.2, A.SSP_RK3(g.shape_fourier(dim), eq), dev(uh))
+    osim = O.Simulation(oeq, uh, 0.2)
+    assert sim.simulate_until(1.0, max_steps=steps) == steps
+    osim.simulate_until(1.0, max_steps=steps)
+    assert abs(sim.time() - osim.time) <= 1e-14
+    assert rel(host(sim.u()), osim.u) <= 1e-12
+
+
 def test_simulate_until_device_loop_matches_host_loop(cuda):
     """The device-driven loop (dt and time on the device) takes the same steps as the
     host loop of Simulation::simulate_until, including the final clamp onto t."""
