@@ -159,6 +159,7 @@ def run_ensemble(args):
            "data": "synthetic", "config": {"workload": "batched 2D ensemble, discontinuous vortex patch "
                                            "(radius 0.2, 16-mode 0.01 perturbation), 96@128^2 + 96@256^2 + "
                                            f"64@512^2, t={t_end}, C=0.2", "samples": sum(counts.values())}}
+    finals = {}
     for precision in (["fp64", "fp32_mixed"] if not args.no_mixed else ["fp64"]):
         log(f"ensemble {precision}: initial data")
         groups = []
@@ -214,6 +215,12 @@ def run_ensemble(args):
             rec["identical_per_sample_results"] = identical
             out["value"] = rec["samples_per_s"]
             out["dtype"] = "f64"
+        finals[precision] = [g.u.clone() for g in groups]
+        if precision == "fp32_mixed" and "fp64" in finals:
+            # every sample at t_end in both precisions: relative L2 over each size group
+            rec["rel_l2_vs_fp64_gpu"] = {
+                str(g.N): float(torch.linalg.vector_norm(b - a) / torch.linalg.vector_norm(a))
+                for g, a, b in zip(groups, finals["fp64"], finals["fp32_mixed"])}
         out[precision] = rec
         ens.close()
     print(json.dumps(out), flush=True)
