@@ -140,7 +140,27 @@ def cpu_baseline(dim: int, budget_s: float):
         "kind": "port",
         "sample": f"{name}, {steps} SSP-RK3 steps in {el:.1f} s, FP64 numpy oracle "
                   f"(pocketfft, not FFTW), 1 thread",
+        "gpu_fp64_vs_oracle": _gpu_vs_oracle(dim, Ns, u0, sim, steps),
     }
+
+
+def _gpu_vs_oracle(dim, Ns, u0, osim, steps):
+    """The FP64 GPU path over the same sample (same initial state, same steps), checked
+    against the oracle's final state: relative L2 over the spectral coefficients."""
+    import numpy as np
+    import torch
+
+    import azeban_amd as A
+
+    g = A.Grid(dim, Ns)
+    eq = A.IncompressibleEuler(g, A.SmoothCutoff1D(0.05 / Ns, 1.0, 2 * math.pi * Ns / 4))
+    sim = A.Simulation(g, 0.2, A.SSP_RK3(g.shape_fourier(dim), eq), torch.from_numpy(u0).to("cuda"))
+    n = sim.simulate_until(1e9, max_steps=steps, chunk=steps)
+    torch.cuda.synchronize()
+    got = sim.u().cpu().numpy()
+    ref = osim.u
+    return {"steps": int(n), "rel_l2": float(np.linalg.norm(got - ref) / np.linalg.norm(ref)),
+            "time_gpu": sim.time(), "time_oracle": osim.time}
 
 
 def run_ensemble(args):
