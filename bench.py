@@ -402,7 +402,7 @@ def main():
                              "rel_l2_vs_fp64_gpu": err,
                              "steps_compared": max(done, args.error_steps),
                              "time_fp64": sa.time(), "time_fp32_mixed": sb.time(),
-                             "kernels": kernm}
+                             "kernels": kernm, "tuning": res["fp32_mixed"]["tuning"]}
     if not args.no_cpu_baseline:
         log("cpu baseline")
         out["cpu_baseline"] = cpu_baseline(dim, args.cpu_seconds)
