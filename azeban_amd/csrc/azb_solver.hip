@@ -33,6 +33,18 @@ int fail(int code, const std::string& msg) {
       return fail(AZB_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
   } while (0)
 
+// device allocation freed when the scope ends, on every return path (hipFree waits
+// for work already enqueued on it)
+struct DevFree {
+  void* p = nullptr;
+  DevFree() = default;
+  DevFree(const DevFree&) = delete;
+  DevFree& operator=(const DevFree&) = delete;
+  ~DevFree() {
+    if (p) (void)hipFree(p);
+  }
+};
+
 bool good_len(int L) {
   switch (L) {
     case 48: case 96: case 192: case 384: case 768: case 1536:
@@ -61,8 +73,11 @@ int make_twiddles(int L, void** out) {
     h[t].x = (T)cosl(a);
     h[t].y = (T)(-sinl(a));
   }
-  HIPCHK(hipMalloc(out, sizeof(cx<T>) * L));
-  HIPCHK(hipMemcpy(*out, h.data(), sizeof(cx<T>) * L, hipMemcpyHostToDevice));
+  DevFree buf;
+  HIPCHK(hipMalloc(&buf.p, sizeof(cx<T>) * L));
+  HIPCHK(hipMemcpy(buf.p, h.data(), sizeof(cx<T>) * L, hipMemcpyHostToDevice));
+  *out = buf.p;
+  buf.p = nullptr;
   return AZB_OK;
 }
 
@@ -881,17 +896,18 @@ template <typename T>
 int fft_T(int dim, int n, int howmany, bool forward, void* in, void* out, hipStream_t st) {
   if (!good_len(n)) return fail(AZB_ERR_UNSUPPORTED, "FFT length " + std::to_string(n) + " not built");
   if (dim < 1 || dim > 3) return fail(AZB_ERR_ARG, "dim must be 1..3");
-  void* tw = nullptr;
-  int rc = make_twiddles<T>(n, &tw);
+  DevFree tw_buf, st_buf;
+  int rc = make_twiddles<T>(n, &tw_buf.p);
   if (rc) return rc;
+  const void* tw = tw_buf.p;
   const long nh = n / 2 + 1;
   long lines = howmany;
   for (int d = 1; d < dim; ++d) lines *= n;
   ColArgs c{};
   c.nsmall = n;
   c.tw = tw;
-  DevState* dst = nullptr;
-  HIPCHK(hipMalloc((void**)&dst, sizeof(DevState)));
+  HIPCHK(hipMalloc(&st_buf.p, sizeof(DevState)));
+  DevState* dst = (DevState*)st_buf.p;
   HIPCHK(hipMemsetAsync(dst, 0, sizeof(DevState), st));
   c.st = dst;
   PlainRowArgs p{};
@@ -930,8 +946,6 @@ int fft_T(int dim, int n, int howmany, bool forward, void* in, void* out, hipStr
   }
   HIPCHK(hipGetLastError());
   HIPCHK(hipStreamSynchronize(st));
-  (void)hipFree(tw);
-  (void)hipFree(dst);
   return AZB_OK;
 }
 }  // namespace
@@ -1001,8 +1015,9 @@ int azb_energy_spectrum(int dim, int N, const void* u_hat, double* spec_host, vo
   hipStream_t st = (hipStream_t)stream;
   const long nh = N / 2 + 1;
   const long nmodes = dim == 2 ? (long)N * nh : (long)N * N * nh;
-  double* d = nullptr;
-  HIPCHK(hipMalloc((void**)&d, sizeof(double) * nh));
+  DevFree buf;
+  HIPCHK(hipMalloc(&buf.p, sizeof(double) * nh));
+  double* d = (double*)buf.p;
   HIPCHK(hipMemsetAsync(d, 0, sizeof(double) * nh, st));
   long blocks = (nmodes + 255) / 256;
   if (blocks > 2048) blocks = 2048;
@@ -1011,7 +1026,6 @@ int azb_energy_spectrum(int dim, int N, const void* u_hat, double* spec_host, vo
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(spec_host, d, sizeof(double) * nh, hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  HIPCHK(hipFree(d));
   return AZB_OK;
 }
 
@@ -1027,8 +1041,9 @@ int azb_axpby(double a, const void* x, double b, void* y, long n, void* stream) 
 int azb_max_norm(const void* x, long n, double* out_host, void* stream) {
   if (!x || !out_host) return fail(AZB_ERR_ARG, "NULL argument");
   hipStream_t st = (hipStream_t)stream;
-  unsigned long long* d = nullptr;
-  HIPCHK(hipMalloc((void**)&d, sizeof(unsigned long long)));
+  DevFree buf;
+  HIPCHK(hipMalloc(&buf.p, sizeof(unsigned long long)));
+  unsigned long long* d = (unsigned long long*)buf.p;
   HIPCHK(hipMemsetAsync(d, 0, sizeof(unsigned long long), st));
   if (n > 0) {
     long blocks = (n + 255) / 256;
@@ -1038,7 +1053,6 @@ int azb_max_norm(const void* x, long n, double* out_host, void* stream) {
   unsigned long long bits = 0;
   HIPCHK(hipMemcpyAsync(&bits, d, sizeof(bits), hipMemcpyDeviceToHost, st));
   HIPCHK(hipStreamSynchronize(st));
-  (void)hipFree(d);
   double v;
   std::memcpy(&v, &bits, sizeof(v));
   *out_host = v;
