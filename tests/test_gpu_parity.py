@@ -227,6 +227,67 @@ def test_dudt_3d_full_size_band_limited(cuda, N, precision, tol):
     assert err <= tol
 
 
+FFT_WORKERS = 16  # host threads for the oracle's 768^3 transforms
+
+
+@pytest.fixture(scope="module")
+def broadband_512():
+    """White noise at 512^3, Leray-projected: every mode up to the Nyquist modes
+    carries data."""
+    print("[case] 3D broadband state N=512", flush=True)
+    rng = np.random.default_rng(7)
+    return O.leray(O.fft_r2c(rng.standard_normal((3, 512, 512, 512)), FFT_WORKERS))
+
+
+def oracle_512():
+    N = 512
+    return O.IncompressibleEuler(O.Grid(3, N), O.SmoothCutoff1D(0.05 / N, 1.0, 2 * math.pi * N / 4),
+                                 workers=FFT_WORKERS)
+
+
+def test_dudt_3d_512_broadband_against_oracle(cuda, broadband_512):
+    """The benchmark's own kernels at full size (M = 768: the radix-24 FP64 row kernel,
+    the 768-point column passes, the k-space kernel) against the oracle on a broadband
+    field.  The oracle runs with scipy.fft's threaded pocketfft (pinned against
+    brute-force DFTs like numpy's); about 30 s and 75 GB of host memory."""
+    N = 512
+    print(f"[case] 3D broadband dudt N={N} fp64 against the oracle", flush=True)
+    uh = broadband_512
+    g, eq, _ = make_pair(3, N)
+    u = dev(uh)
+    out = torch.empty_like(u)
+    eq.dudt(out, u)
+    got = host(out)
+    del u, out
+    print("[case] GPU RHS done; oracle RHS", flush=True)
+    oeq = oracle_512()
+    ref = oeq.dudt(uh)
+    err = rel(got, ref)
+    print(f"3D broadband N={N}: rel L2 {err:.3e}", flush=True)
+    assert err <= 1e-12, err
+    assert abs(eq.u_max() - oeq.u_max) <= 1e-14 * oeq.u_max
+
+
+def test_rk3_step_3d_512_broadband_against_oracle(cuda, broadband_512):
+    """One SSP-RK3 step at 512^3 (three RHS, the fused stage updates, the device CFL
+    step) against the oracle's step from the same broadband state."""
+    N = 512
+    print(f"[case] 3D broadband RK3 step N={N} fp64 against the oracle", flush=True)
+    uh = broadband_512
+    g, eq, _ = make_pair(3, N)
+    rk = A.SSP_RK3(g.shape_fourier(3), eq)
+    u = dev(uh)
+    dt = rk.integrate(1.0, 1.0, 0.2, u)
+    got = host(u)
+    del u, rk
+    print("[case] GPU step done; oracle step (three RHS)", flush=True)
+    ref, dt_ref = O.ssp_rk3_integrate(oracle_512(), uh, 1.0, 0.2)
+    err = rel(got, ref)
+    print(f"3D broadband RK3 step N={N}: dt {dt:.6e} rel L2 {err:.3e}", flush=True)
+    assert abs(dt - dt_ref) <= 1e-14 * dt_ref
+    assert err <= 1e-12, err
+
+
 def test_row_kernel_tiles_identical_at_full_size(cuda):
     """At M = 768 the FP64 row kernel runs with one line pair per workgroup (the
     default) or two (pdiv = 1), and the column passes with tiles of 8, 4 or 2
