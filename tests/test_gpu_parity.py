@@ -424,6 +424,64 @@ def test_trajectory_large_grids(cuda, dim, N, steps, ic):
     assert rel(host(sim.u()), osim.u) <= 1e-12
 
 
+def test_trajectory_dsl_1024_100_steps(cuda):
+    """The 2D double-shear-layer configuration at its full size: N = 1024^2, FP64, 100
+    SSP-RK3 steps of the device-driven loop against the oracle (threaded transforms):
+    relative L2 <= 1e-11 in velocity, the same time and energy spectrum, divergence at
+    round-off."""
+    N, steps = 1024, 100
+    print(f"[case] DSL N={N} {steps} steps against the oracle", flush=True)
+    uh = O.initialize(I.double_shear_layer(N, seed=1))
+    g, eq, _ = make_pair(2, N)
+    sim = A.Simulation(g, 0.2, A.SSP_RK3(g.shape_fourier(2), eq), dev(uh))
+    assert sim.simulate_until(1.0, max_steps=steps) == steps
+    got = host(sim.u())
+    oeq = O.IncompressibleEuler(O.Grid(2, N), O.SmoothCutoff1D(0.05 / N, 1.0, 2 * math.pi * N / 4),
+                                workers=FFT_WORKERS)
+    osim = O.Simulation(oeq, uh, 0.2)
+    for done in range(20, steps + 1, 20):
+        osim.simulate_until(1.0, max_steps=20)
+        print(f"oracle: {done} steps, t = {osim.time:.6e}", flush=True)
+    assert len(osim.dts) == steps
+    assert abs(sim.time() - osim.time) <= 1e-12 * osim.time
+    err = rel(got, osim.u)
+    print(f"DSL N={N} after {steps} steps: rel L2 {err:.3e}", flush=True)
+    assert err <= 1e-11, err
+    ek = np.asarray(A.energy_spectrum(sim.u()))
+    ek_ref = O.energy_spectrum(osim.u, N)
+    assert np.allclose(ek, ek_ref, rtol=1e-10, atol=1e-12 * float(np.sum(ek_ref)))
+    assert spectral_divergence(got) <= 1e-13
+
+
+@pytest.mark.timeout(400)  # about two minutes, most of it the oracle's 300 RHS
+def test_trajectory_3d_fbm_128_100_steps(cuda):
+    """A broadband 3D state (fractional Brownian, H = 0.75) at 128^3, 100 SSP-RK3 steps
+    of the device-driven loop against the oracle (threaded transforms): relative L2
+    <= 1e-11 in velocity, the same time and energy spectrum, divergence at round-off."""
+    N, steps = 128, 100
+    uh = random_state(3, N, seed=9)
+    print(f"[case] 3D fBm N={N} {steps} steps against the oracle", flush=True)
+    g, eq, _ = make_pair(3, N)
+    sim = A.Simulation(g, 0.2, A.SSP_RK3(g.shape_fourier(3), eq), dev(uh))
+    assert sim.simulate_until(1.0, max_steps=steps) == steps
+    got = host(sim.u())
+    oeq = O.IncompressibleEuler(O.Grid(3, N), O.SmoothCutoff1D(0.05 / N, 1.0, 2 * math.pi * N / 4),
+                                workers=FFT_WORKERS)
+    osim = O.Simulation(oeq, uh, 0.2)
+    for done in range(20, steps + 1, 20):
+        osim.simulate_until(1.0, max_steps=20)
+        print(f"oracle: {done} steps, t = {osim.time:.6e}", flush=True)
+    assert len(osim.dts) == steps
+    assert abs(sim.time() - osim.time) <= 1e-12 * osim.time
+    err = rel(got, osim.u)
+    print(f"3D fBm N={N} after {steps} steps: rel L2 {err:.3e}", flush=True)
+    assert err <= 1e-11, err
+    ek = np.asarray(A.energy_spectrum(sim.u()))
+    ek_ref = O.energy_spectrum(osim.u, N)
+    assert np.allclose(ek, ek_ref, rtol=1e-10, atol=1e-12 * float(np.sum(ek_ref)))
+    assert spectral_divergence(got) <= 1e-13
+
+
 def test_simulate_until_device_loop_matches_host_loop(cuda):
     """The device-driven loop (dt and time on the device) takes the same steps as the
     host loop of Simulation::simulate_until, including the final clamp onto t."""
