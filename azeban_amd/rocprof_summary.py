@@ -3,6 +3,8 @@
     python -m azeban_amd.rocprof_summary trace  <kt_results.db>  --title T --cmd C
     python -m azeban_amd.rocprof_summary counters --fetch F.db --write W.db [--sq S.db ...]
                                           --title T --cmd C --json out.json
+    python -m azeban_amd.rocprof_summary dispatches <pmc_results.db> --title T --cmd C
+                                          [--match azb:: --limit 12]
 
 Counter runs hold one pass per database (FETCH_SIZE and WRITE_SIZE need their own
 runs).  The HBM bytes of a dispatch are FETCH_SIZE x fetch_scale + WRITE_SIZE x
@@ -84,7 +86,29 @@ def main(argv=None) -> int:
     c.add_argument("--cmd", required=True)
     c.add_argument("--json")
     c.add_argument("--match", default="azb::", help="only kernels whose name contains this")
+    d = sub.add_parser("dispatches")
+    d.add_argument("db")
+    d.add_argument("--title", required=True)
+    d.add_argument("--cmd", required=True)
+    d.add_argument("--match", default="azb::", help="only kernels whose name contains this")
+    d.add_argument("--limit", type=int, default=12, help="first dispatches shown")
     a = ap.parse_args(argv)
+
+    if a.mode == "dispatches":
+        # one row per dispatch, in launch order: tells apart kernels that share one
+        # template instantiation but run different passes
+        ev = [e for e in counter_events(a.db) if a.match in e["name"]][: a.limit]
+        names = sorted({k for e in ev for k in e if k not in ("name", "dur_ns")})
+        print(f"# {a.title}\n")
+        print("Command (from the repository root, one MI355X):\n")
+        print(f"```\n{a.cmd}\n```\n")
+        print("Per dispatch, in launch order (counters summed over the device):\n")
+        print("| # | kernel | us | " + " | ".join(names) + " |")
+        print("|---|---|---|" + "---|" * len(names))
+        for i, e in enumerate(ev):
+            vals = " | ".join(f"{e.get(n, 0):.0f}" for n in names)
+            print(f"| {i} | `{_kernel_short(e['name'])}` | {e['dur_ns'] / 1e3:.1f} | {vals} |")
+        return 0
 
     if a.mode == "trace":
         print(f"# {a.title}\n")
