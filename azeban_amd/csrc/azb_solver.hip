@@ -954,6 +954,8 @@ extern "C" {
 
 int azb_fft_r2c(int dim, int n, int howmany, int precision, const void* in, void* out, void* stream) {
   if (!in || !out) return fail(AZB_ERR_ARG, "NULL argument");
+  if (precision != AZB_FP64 && precision != AZB_FP32_MIXED) return fail(AZB_ERR_ARG, "unknown precision");
+  if (howmany < 1) return fail(AZB_ERR_ARG, "howmany must be positive");
   hipStream_t st = (hipStream_t)stream;
   return precision == AZB_FP64 ? fft_T<double>(dim, n, howmany, true, (void*)in, out, st)
                                : fft_T<float>(dim, n, howmany, true, (void*)in, out, st);
@@ -961,6 +963,8 @@ int azb_fft_r2c(int dim, int n, int howmany, int precision, const void* in, void
 
 int azb_fft_c2r(int dim, int n, int howmany, int precision, void* in, void* out, void* stream) {
   if (!in || !out) return fail(AZB_ERR_ARG, "NULL argument");
+  if (precision != AZB_FP64 && precision != AZB_FP32_MIXED) return fail(AZB_ERR_ARG, "unknown precision");
+  if (howmany < 1) return fail(AZB_ERR_ARG, "howmany must be positive");
   hipStream_t st = (hipStream_t)stream;
   return precision == AZB_FP64 ? fft_T<double>(dim, n, howmany, false, in, out, st)
                                : fft_T<float>(dim, n, howmany, false, in, out, st);
@@ -1001,6 +1005,7 @@ int azb_scale(double a, void* x, long n, void* stream) {
 int azb_leray(int dim, int N, void* u_hat, void* stream) {
   if (!u_hat) return fail(AZB_ERR_ARG, "NULL argument");
   if (dim != 2 && dim != 3) return fail(AZB_ERR_ARG, "dim must be 2 or 3");
+  if (N < 2 || N % 2) return fail(AZB_ERR_ARG, "N must be even and positive");
   const long nh = N / 2 + 1;
   const long nmodes = dim == 2 ? (long)N * nh : (long)N * N * nh;
   hipLaunchKernelGGL(leray_kernel, dim3((unsigned)((nmodes + 255) / 256)), dim3(256), 0,
@@ -1012,6 +1017,7 @@ int azb_leray(int dim, int N, void* u_hat, void* stream) {
 int azb_energy_spectrum(int dim, int N, const void* u_hat, double* spec_host, void* stream) {
   if (!u_hat || !spec_host) return fail(AZB_ERR_ARG, "NULL argument");
   if (dim != 2 && dim != 3) return fail(AZB_ERR_ARG, "dim must be 2 or 3");
+  if (N < 2 || N % 2) return fail(AZB_ERR_ARG, "N must be even and positive");
   hipStream_t st = (hipStream_t)stream;
   const long nh = N / 2 + 1;
   const long nmodes = dim == 2 ? (long)N * nh : (long)N * N * nh;

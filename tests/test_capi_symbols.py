@@ -82,3 +82,26 @@ def test_entry_points_reject_null_handles():
     assert lib.azb_copy_from_padded(4, 24, 16, C.c_void_p(8), C.c_void_p(8), None) == _capi.AZB_ERR_ARG
     assert lib.azb_scale(1.0, None, 4, None) == _capi.AZB_ERR_ARG
     assert "NULL" in lib.azb_last_error().decode() or lib.azb_last_error()
+
+
+def test_transform_and_helper_entry_points_reject_bad_arguments():
+    """Argument checks that return before any device call (no GPU needed)."""
+    lib = _capi.load()
+    p = C.c_void_p(8)
+    out = C.c_double()
+    spec = (C.c_double * 8)()
+    for fn in (lib.azb_fft_r2c, lib.azb_fft_c2r):
+        assert fn(3, 64, 1, 0, None, p, None) == _capi.AZB_ERR_ARG
+        assert fn(3, 64, 1, 7, p, p, None) == _capi.AZB_ERR_ARG
+        assert "precision" in lib.azb_last_error().decode()
+        assert fn(3, 64, 0, 0, p, p, None) == _capi.AZB_ERR_ARG
+        assert fn(3, 100, 1, 0, p, p, None) == _capi.AZB_ERR_UNSUPPORTED
+        assert fn(4, 64, 1, 0, p, p, None) == _capi.AZB_ERR_ARG
+    assert lib.azb_leray(2, 64, None, None) == _capi.AZB_ERR_ARG
+    assert lib.azb_leray(4, 64, p, None) == _capi.AZB_ERR_ARG
+    assert lib.azb_leray(2, 0, p, None) == _capi.AZB_ERR_ARG
+    assert lib.azb_energy_spectrum(2, 64, None, spec, None) == _capi.AZB_ERR_ARG
+    assert lib.azb_energy_spectrum(4, 64, p, spec, None) == _capi.AZB_ERR_ARG
+    assert lib.azb_energy_spectrum(2, 7, p, spec, None) == _capi.AZB_ERR_ARG
+    assert lib.azb_max_norm(None, 4, C.byref(out), None) == _capi.AZB_ERR_ARG
+    assert lib.azb_axpby(1.0, None, 1.0, p, 4, None) == _capi.AZB_ERR_ARG
