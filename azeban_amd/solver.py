@@ -38,6 +38,38 @@ def _stream() -> C.c_void_p:
     return C.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def _need(name: str, t: torch.Tensor, dtypes, shape_ok=None, what: str = "") -> None:
+    """Reject a tensor of the wrong dtype or shape before any library call: the entry
+    points take a bare pointer and an element count, so a mismatch would read or write
+    outside the tensor on the device."""
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name}: expected a torch tensor")
+    if t.dtype not in dtypes:
+        raise ValueError(f"{name}: expected dtype {' or '.join(map(str, dtypes))}, got {t.dtype}")
+    if shape_ok is not None and not shape_ok(tuple(t.shape)):
+        raise ValueError(f"{name}: expected shape {what}, got {tuple(t.shape)}")
+
+
+def _fourier_shape_ok(shape) -> bool:
+    """(dim, N, ..., N/2+1) with dim 2 or 3 and an even N."""
+    if len(shape) not in (3, 4) or shape[0] != len(shape) - 1:
+        return False
+    N = shape[1]
+    return N >= 2 and N % 2 == 0 and all(n == N for n in shape[1:-1]) and shape[-1] == N // 2 + 1
+
+
+def _component_shape_ok(shape) -> bool:
+    """One component (N, ..., N/2+1) in 2 or 3 dimensions with an even N."""
+    if len(shape) not in (2, 3):
+        return False
+    N = shape[0]
+    return N >= 2 and N % 2 == 0 and all(n == N for n in shape[:-1]) and shape[-1] == N // 2 + 1
+
+
+_FOURIER = "(dim, N, ..., N/2+1) with dim = number of axes after the first"
+_COMPONENT = "(N, ..., N/2+1) in 2 or 3 dimensions"
+
+
 @dataclass
 class Grid:
     """Grid<Dim>: N_phys, N_phys_pad (default 3N/2), N_fourier = N/2+1."""
@@ -273,6 +305,8 @@ class Simulation:
 # ---------------------------------------------------------------------------
 def fft_r2c(u: torch.Tensor) -> torch.Tensor:
     """Unnormalised R2C over all but the leading (batch) axis."""
+    _need("fft_r2c", u, (torch.float64, torch.float32),
+          lambda sh: 2 <= len(sh) <= 4 and sh[0] >= 1 and len(set(sh[1:])) == 1, "(howmany, n, ..., n)")
     dim = u.dim() - 1
     n = u.shape[-1]
     prec = AZB_FP64 if u.dtype == torch.float64 else AZB_FP32_MIXED
@@ -285,6 +319,9 @@ def fft_r2c(u: torch.Tensor) -> torch.Tensor:
 
 def fft_c2r(u_hat: torch.Tensor, n: int) -> torch.Tensor:
     """Unnormalised C2R; works on a copy (the library overwrites its input)."""
+    _need("fft_c2r", u_hat, (torch.complex128, torch.complex64),
+          lambda sh: 2 <= len(sh) <= 4 and sh[0] >= 1 and all(x == n for x in sh[1:-1]) and sh[-1] == n // 2 + 1,
+          f"(howmany, {n}, ..., {n // 2 + 1})")
     dim = u_hat.dim() - 1
     prec = AZB_FP64 if u_hat.dtype == torch.complex128 else AZB_FP32_MIXED
     rdt = torch.float64 if u_hat.dtype == torch.complex128 else torch.float32
@@ -295,6 +332,7 @@ def fft_c2r(u_hat: torch.Tensor, n: int) -> torch.Tensor:
 
 
 def leray_(u_hat: torch.Tensor) -> torch.Tensor:
+    _need("leray_", u_hat, (torch.complex128,), _fourier_shape_ok, _FOURIER)
     dim = u_hat.dim() - 1
     check(load().azb_leray(dim, u_hat.shape[1], _ptr(u_hat), _stream()))
     return u_hat
@@ -307,6 +345,7 @@ def initialize(u_phys) -> torch.Tensor:
 
 
 def energy_spectrum(u_hat: torch.Tensor):
+    _need("energy_spectrum", u_hat, (torch.complex128,), _fourier_shape_ok, _FOURIER)
     dim = u_hat.dim() - 1
     N = u_hat.shape[1]
     spec = (C.c_double * (N // 2 + 1))()
@@ -316,6 +355,9 @@ def energy_spectrum(u_hat: torch.Tensor):
 
 def copy_to_padded(src: torch.Tensor, M: int) -> torch.Tensor:
     """One component, N -> M zero padding."""
+    _need("copy_to_padded", src, (torch.complex128,), _component_shape_ok, _COMPONENT)
+    if M < src.shape[0]:
+        raise ValueError("copy_to_padded: M must not be smaller than N")
     dim = src.dim()
     N = src.shape[0]
     shape = (M,) * (dim - 1) + (M // 2 + 1,)
@@ -327,6 +369,9 @@ def copy_to_padded(src: torch.Tensor, M: int) -> torch.Tensor:
 
 def copy_from_padded(src: torch.Tensor, N: int) -> torch.Tensor:
     """One component, M -> N truncation (the inverse of copy_to_padded)."""
+    _need("copy_from_padded", src, (torch.complex128,), _component_shape_ok, _COMPONENT)
+    if not 2 <= N <= src.shape[0] or N % 2:
+        raise ValueError("copy_from_padded: N must be even and at most the padded size")
     dim = src.dim()
     M = src.shape[0]
     shape = (N,) * (dim - 1) + (N // 2 + 1,)
@@ -364,11 +409,18 @@ def to_physical(u_hat: torch.Tensor, N_out: int | None = None) -> torch.Tensor:
 
 
 def axpby_(a: float, x: torch.Tensor, b: float, y: torch.Tensor) -> torch.Tensor:
+    """y <- a x + b y over complex128 tensors of equal size."""
+    _need("axpby_ x", x, (torch.complex128,))
+    _need("axpby_ y", y, (torch.complex128,))
+    if x.numel() != y.numel():
+        raise ValueError(f"axpby_: x has {x.numel()} elements, y {y.numel()}")
     check(load().azb_axpby(a, _ptr(x), b, _ptr(y), x.numel(), _stream()))
     return y
 
 
 def max_norm(x: torch.Tensor) -> float:
+    """max |x| over a float64 tensor (the reference's max_norm of a real array)."""
+    _need("max_norm", x, (torch.float64,))
     out = C.c_double()
     x = x.contiguous()
     check(load().azb_max_norm(_ptr(x), x.numel(), C.byref(out), _stream()))

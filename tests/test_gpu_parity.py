@@ -147,6 +147,40 @@ def test_energy_spectrum(cuda, dim, N):
     assert np.allclose(got, ref, rtol=1e-12, atol=1e-300)
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+def test_energy_spectrum_single_modes_gpu(cuda, dim):
+    """The device shell sums on the hand-written single-mode cases of the reference's
+    rule (L-infinity shells, k_last != 0 doubled including the Nyquist index): the
+    same known answers the oracle is pinned with, exactly."""
+    from test_oracle import _check_single_modes
+
+    print(f"[case] energy spectrum single modes dim={dim}", flush=True)
+    _check_single_modes(lambda uh, N: np.array(A.energy_spectrum(dev(uh))), dim)
+
+
+def test_golden_fixture_2d_gpu(cuda):
+    """The committed fixture (written by tests/golden/make_golden.py from the oracle):
+    the HIP RHS of its state and its 5-step SSP-RK3 trajectory and dt sequence."""
+    import os
+
+    print("[case] golden fixture 2D N=64", flush=True)
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "golden_2d_n64.npz"))
+    N = 64
+    g, eq, _ = make_pair(2, N)
+    u = dev(fx["u0"])
+    d = torch.empty_like(u)
+    eq.dudt(d, u)
+    assert rel(host(d), fx["dudt"]) <= 1e-12
+    sim = A.Simulation(g, 0.2, A.SSP_RK3(g.shape_fourier(2), eq), u)
+    dts = []
+    for _ in range(5):
+        t0 = sim.time()
+        sim.simulate_until(1.0, max_steps=1, chunk=1)
+        dts.append(sim.time() - t0)
+    assert rel(host(sim.u()), fx["u5"]) <= 1e-12
+    assert np.allclose(dts, fx["dts"], rtol=1e-12, atol=0)
+
+
 # ---------------------------------------------------------------------------
 # RHS (IncompressibleEuler::dudt)
 # ---------------------------------------------------------------------------
@@ -523,7 +557,9 @@ def test_simulate_until_edge_cases(cuda):
     assert n == 3 and t_small < sim.time() < 1.0
 
 
-@pytest.mark.parametrize("dim,N,steps", [(2, 256, 20), (3, 64, 10)])
+# every FP32 row-kernel path: 2D L = 384, 768 (row-2 kernel, FSPLIT off), 1536 (the
+# unsplit FP32 forward exchange), 3D L = 96 and 192 (row-4 kernel)
+@pytest.mark.parametrize("dim,N,steps", [(2, 256, 20), (2, 512, 10), (2, 1024, 10), (3, 64, 10), (3, 128, 5)])
 def test_fp32_mixed_error(cuda, dim, N, steps):
     print(f"[case] fp32_mixed dim={dim} N={N}", flush=True)
     uh = random_state(dim, N, seed=6)

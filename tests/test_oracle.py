@@ -328,6 +328,92 @@ def test_energy_spectrum_parseval(dim, N):
     assert len(e) == N // 2 + 1
 
 
+# Single-mode known answers of the reference's shell rule
+# (include/azeban/operations/spectrum.hpp:33-118, energy_spectrum.cpp:10-50): a mode
+# stored at row index i (signed k = i - N for i >= N/2 + 1, so index N/2 is +N/2) and
+# last-axis index k (0 <= k <= N/2, never signed) lands in the L-infinity shell
+# K = max(|k_i|), and counts 1/2 |u_hat / N^d|^2 once for k = 0 and twice (its
+# conjugate partner) for every k != 0, the Nyquist index k = N/2 included.
+# Each case: (index tuple, component, shell K, weight).  The shells are written out by
+# hand; an L2 shell rint(|k|) differs on every oblique case ((-3, 3) -> 4, (2, -5, 1) -> 5
+# vs 6, ...), and dropping the Nyquist doubling changes the weight of the k = N/2 cases.
+SPECTRUM_CASES = {
+    2: (16, [((13, 3), 0, 3, 2.0),     # k = (-3, 3): L-inf 3, L2 4
+             ((5, 0), 1, 5, 1.0),      # k_last = 0: no conjugate partner counted
+             ((2, 8), 0, 8, 2.0),      # k_last = N/2 (Nyquist) is doubled too
+             ((8, 1), 1, 8, 2.0),      # row index N/2 is +N/2
+             ((9, 2), 0, 7, 2.0),      # row index N/2 + 1 is -7
+             ((15, 7), 1, 7, 2.0),     # k = (-1, 7): L2 7.07 -> 7 too, weight 2
+             ((0, 0), 0, 0, 1.0)]),
+    3: (16, [((2, 11, 1), 0, 5, 2.0),  # k = (2, -5, 1): L-inf 5, L2 rint(5.48) = 5
+             ((12, 3, 0), 2, 4, 1.0),  # k = (-4, 3, 0): L2 5, no doubling
+             ((1, 2, 8), 1, 8, 2.0),   # Nyquist k_last doubled
+             ((8, 0, 0), 0, 8, 1.0),   # +N/2 along axis 0, k_last = 0
+             ((13, 13, 3), 2, 3, 2.0),  # k = (-3, -3, 3): L2 rint(5.2) = 5
+             ((0, 9, 6), 1, 7, 2.0),   # k = (0, -7, 6): L2 rint(9.2) = 9
+             ((0, 0, 0), 1, 0, 1.0)]),
+}
+
+
+def _check_single_modes(spectrum, dim):
+    """Every SPECTRUM_CASES entry through `spectrum(u_hat, N)`; raises AssertionError on
+    the first mismatch."""
+    N, cases = SPECTRUM_CASES[dim]
+    a = 3.0 - 4.0j  # |a|^2 = 25
+    for idx, comp, K, w in cases:
+        uh = np.zeros((dim,) + (N,) * (dim - 1) + (N // 2 + 1,), dtype=np.complex128)
+        uh[(comp,) + idx] = a * N**dim  # |u_hat / N^d|^2 = 25
+        e = spectrum(uh, N)
+        want = np.zeros(N // 2 + 1)
+        want[K] = 0.5 * 25.0 * w
+        assert np.array_equal(e, want), (idx, comp, K, w, e)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_energy_spectrum_single_modes(dim):
+    _check_single_modes(O.energy_spectrum, dim)
+
+
+def test_energy_spectrum_single_modes_reject_other_shell_rules():
+    """The single-mode cases fail an L2-shell, no-Nyquist-doubling spectrum (the
+    variant the Parseval test cannot tell apart), so they pin the shell rule."""
+
+    def l2_shells(uh, N):
+        dim = uh.ndim - 1
+        kf = O._signed(np.arange(N), N)
+        kl = np.arange(uh.shape[-1])
+        ks = np.meshgrid(*([kf] * (dim - 1) + [kl]), indexing="ij")
+        K = np.rint(np.sqrt(sum(k.astype(float) ** 2 for k in ks))).astype(int)
+        w = np.where((ks[-1] != 0) & (ks[-1] != N // 2), 2.0, 1.0)
+        e = 0.5 * np.sum(np.abs(uh / float(N) ** dim) ** 2, axis=0) * w
+        spec = np.zeros(N // 2 + 1)
+        keep = K <= N // 2
+        np.add.at(spec, K[keep], e[keep])
+        return spec
+
+    for dim in (2, 3):
+        with pytest.raises(AssertionError):
+            _check_single_modes(l2_shells, dim)
+
+
+def test_energy_spectrum_of_physical_fields():
+    """Interior modes carry the physical energy: u = cos(2 pi (3x - 2y)) has
+    1/2 <u^2> = 1/4, all in shell 3; k_last = 0 splits over the two stored partners
+    (u = cos(2 pi 5 x): 1/4 in shell 5).  The Nyquist line u = (-1)^j along the last
+    axis has 1/2 <u^2> = 1/2 but is stored once and doubled, so the reference's rule
+    reports 1 in shell N/2."""
+    N = 16
+    x = np.arange(N) / N
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    for u, K, want in [(np.cos(2 * np.pi * (3 * X - 2 * Y)), 3, 0.25),
+                       (np.cos(2 * np.pi * 5 * X), 5, 0.25),
+                       (np.cos(np.pi * N * Y), N // 2, 1.0)]:
+        field = np.stack([u, np.zeros_like(u)])
+        e = O.energy_spectrum(O.fft_r2c(field), N)
+        assert e[K] == pytest.approx(want, rel=1e-13)
+        assert np.sum(np.abs(np.delete(e, K))) <= 1e-25
+
+
 # --------------------------------------------------------------------------- golden
 def test_golden_fixture_2d():
     """The committed fixture (tests/golden/make_golden.py) still matches the oracle."""
