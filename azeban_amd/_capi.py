@@ -46,6 +46,7 @@ SIGNATURES = {
     "azb_num_samples": (_i, [_vp]),
     "azb_bytes_estimate": (_sz, [_i, _i, _i, _i]),
     "azb_loop_begin": (_i, [_vp, _d, _d, _dp]),
+    "azb_loop_begin_sample": (_i, [_vp, _i, _d, _d, _d]),
     "azb_enqueue_steps": (_i, [_vp, _vp, _i]),
     "azb_loop_read": (_i, [_vp, _dp, _ip]),
     "azb_loop_probe": (_i, [_vp]),
@@ -59,6 +60,7 @@ SIGNATURES = {
     "azb_stream_create": (_i, [C.POINTER(_vp)]),
     "azb_stream_destroy": (_i, [_vp]),
     "azb_stream_synchronize": (_i, [_vp]),
+    "azb_stream_wait": (_i, [_vp, _vp]),
     "azb_workspace_bytes": (_sz, [_vp]),
     "azb_stream": (_vp, [_vp]),
     "azb_dudt": (_i, [_vp, _vp, _vp]),
@@ -68,6 +70,10 @@ SIGNATURES = {
     "azb_simulate_until": (_i, [_vp, _vp, _d, _d, _dp, _i, _i, _ip]),
     "azb_fft_r2c": (_i, [_i, _i, _i, _i, _vp, _vp, _vp]),
     "azb_fft_c2r": (_i, [_i, _i, _i, _i, _vp, _vp, _vp]),
+    "azb_fft_work_bytes": (_sz, [_i, _i]),
+    "azb_fft_work_init": (_i, [_vp, _i, _i]),
+    "azb_fft_r2c_ws": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp]),
+    "azb_fft_c2r_ws": (_i, [_i, _i, _i, _i, _vp, _vp, _vp, _vp]),
     "azb_copy_to_padded": (_i, [_i, _i, _i, _vp, _vp, _vp]),
     "azb_copy_from_padded": (_i, [_i, _i, _i, _vp, _vp, _vp]),
     "azb_scale": (_i, [_d, _vp, _l, _vp]),
@@ -80,6 +86,7 @@ SIGNATURES = {
     "azb_free": (_i, [_vp]),
     "azb_memcpy_h2d": (_i, [_vp, _vp, _sz]),
     "azb_memcpy_d2h": (_i, [_vp, _vp, _sz]),
+    "azb_memcpy_d2d": (_i, [_vp, _vp, _sz]),
     "azb_profile_enable": (_i, [_vp, _i]),
     "azb_profile_read": (_i, [_vp, _dp, C.POINTER(C.c_long), _i]),
     "azb_profile_reset": (_i, [_vp]),

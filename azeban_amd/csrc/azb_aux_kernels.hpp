@@ -41,6 +41,17 @@ __global__ void step_epilogue_loop(DevState* st_all) {
   }
 }
 
+// (re)start the device-driven loop of one sample: its time, target and step count
+__global__ void loop_set_sample(DevState* st_all, int sample, double time, double t_end, double max_dt_visc) {
+  if (threadIdx.x != 0) return;
+  DevState* st = st_all + sample;
+  st->time = time;
+  st->t_end = t_end;
+  st->max_dt_visc = max_dt_visc;
+  st->done = time < t_end ? 0 : 1;
+  st->steps = 0;
+}
+
 // reduce one slot's bins: u_max = max / M^d; stage 1 also sets dt = min(C/(N u_max), max_dt)
 __global__ void finalize_umax(DevState* st_all, int slot, int set_dt, double Md, double C, double N) {
   DevState* st = st_all + blockIdx.x;

@@ -65,7 +65,7 @@ int row_p(int L) {
 // twiddle tables exp(-2 pi i t / L), computed in long double on the host
 // ---------------------------------------------------------------------------
 template <typename T>
-int make_twiddles(int L, void** out) {
+std::vector<cx<T>> host_twiddles(int L) {
   std::vector<cx<T>> h(L);
   const long double twopi = 6.283185307179586476925286766559005768L;
   for (int t = 0; t < L; ++t) {
@@ -73,6 +73,12 @@ int make_twiddles(int L, void** out) {
     h[t].x = (T)cosl(a);
     h[t].y = (T)(-sinl(a));
   }
+  return h;
+}
+
+template <typename T>
+int make_twiddles(int L, void** out) {
+  const std::vector<cx<T>> h = host_twiddles<T>(L);
   DevFree buf;
   HIPCHK(hipMalloc(&buf.p, sizeof(cx<T>) * L));
   HIPCHK(hipMemcpy(buf.p, h.data(), sizeof(cx<T>) * L, hipMemcpyHostToDevice));
@@ -200,6 +206,7 @@ struct azb_solver {
   size_t bytes = 0;
   long nmodes = 0;  // modes per component on the N-grid
   double loop_C = 0.2;  // CFL number of the device-driven loop
+  bool loop_begun = false;  // loop_C fixed by azb_loop_begin / azb_loop_begin_sample
   // tuning: column tile divisor per kernel family, line-pair divisor of the row kernel
   int tsdiv[AZB_PROF_FAMILIES] = {1, 1, 1, 1, 1, 1};
   int pdiv = 1;
@@ -550,8 +557,24 @@ int azb_loop_begin(azb_solver* s, double t_end, double C, const double* times0_h
     h[i].steps = 0;
   }
   s->loop_C = C;
+  s->loop_begun = true;
   HIPCHK(hipMemcpyAsync(s->st, h.data(), s->S * sizeof(DevState), hipMemcpyHostToDevice, s->stream));
   HIPCHK(hipStreamSynchronize(s->stream));
+  return AZB_OK;
+}
+
+int azb_loop_begin_sample(azb_solver* s, int sample, double t_end, double C, double time0) {
+  if (!s) return fail(AZB_ERR_ARG, "NULL solver");
+  if (sample < 0 || sample >= s->S)
+    return fail(AZB_ERR_ARG, "sample " + std::to_string(sample) + " outside [0, " + std::to_string(s->S) + ")");
+  if (C != s->loop_C && s->loop_begun)
+    return fail(AZB_ERR_ARG, "every sample of a solver shares one CFL number");
+  const double pi = 3.14159265358979323846;
+  s->loop_C = C;
+  s->loop_begun = true;
+  hipLaunchKernelGGL(loop_set_sample, dim3(1), dim3(64), 0, s->stream, s->st, sample, time0, t_end,
+                     C * 2. / (s->visc.eps * ((pi * s->N) * (pi * s->N))));
+  HIPCHK(hipGetLastError());
   return AZB_OK;
 }
 
@@ -707,6 +730,18 @@ int azb_stream_destroy(void* stream) {
 
 int azb_stream_synchronize(void* stream) {
   HIPCHK(hipStreamSynchronize((hipStream_t)stream));
+  return AZB_OK;
+}
+
+int azb_stream_wait(void* waiter, void* waitee) {
+  if (waiter == waitee) return AZB_OK;
+  hipEvent_t e = nullptr;
+  HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  hipError_t r = hipEventRecord(e, (hipStream_t)waitee);
+  if (r == hipSuccess) r = hipStreamWaitEvent((hipStream_t)waiter, e, 0);
+  // the wait holds what it needs: the event may be released at once
+  (void)hipEventDestroy(e);
+  if (r != hipSuccess) return fail(AZB_ERR_HIP, std::string("azb_stream_wait: ") + hipGetErrorString(r));
   return AZB_OK;
 }
 
@@ -885,6 +920,10 @@ int azb_memcpy_d2h(void* dst_host, const void* src, size_t bytes) {
   HIPCHK(hipMemcpy(dst_host, src, bytes, hipMemcpyDeviceToHost));
   return AZB_OK;
 }
+int azb_memcpy_d2d(void* dst, const void* src, size_t bytes) {
+  HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
+  return AZB_OK;
+}
 
 }  // extern "C"
 
@@ -892,27 +931,41 @@ int azb_memcpy_d2h(void* dst_host, const void* src, size_t bytes) {
 // standalone seams
 // ---------------------------------------------------------------------------
 namespace {
+// work area of a standalone transform of length n: the device state the column
+// kernels read (never marked done) at offset 0, the twiddle table after it
+constexpr size_t kFftTwOffset = (sizeof(DevState) + 255) / 256 * 256;
+
 template <typename T>
-int fft_T(int dim, int n, int howmany, bool forward, void* in, void* out, hipStream_t st) {
+size_t fft_work_bytes(int n) {
+  return kFftTwOffset + (sizeof(cx<T>) * (size_t)n + 255) / 256 * 256;
+}
+
+template <typename T>
+int fft_work_init(void* work, int n) {
+  const std::vector<cx<T>> h = host_twiddles<T>(n);
+  std::vector<unsigned char> head(kFftTwOffset, 0);
+  HIPCHK(hipMemcpy(work, head.data(), kFftTwOffset, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy((char*)work + kFftTwOffset, h.data(), sizeof(cx<T>) * n, hipMemcpyHostToDevice));
+  return AZB_OK;
+}
+
+// queue one rank-dim transform; no allocation, no synchronisation
+template <typename T>
+int fft_ws_T(int dim, int n, int howmany, bool forward, void* in, void* out, void* work, hipStream_t st) {
   if (!good_len(n)) return fail(AZB_ERR_UNSUPPORTED, "FFT length " + std::to_string(n) + " not built");
   if (dim < 1 || dim > 3) return fail(AZB_ERR_ARG, "dim must be 1..3");
-  DevFree tw_buf, st_buf;
-  int rc = make_twiddles<T>(n, &tw_buf.p);
-  if (rc) return rc;
-  const void* tw = tw_buf.p;
+  const void* tw = (const char*)work + kFftTwOffset;
   const long nh = n / 2 + 1;
   long lines = howmany;
   for (int d = 1; d < dim; ++d) lines *= n;
   ColArgs c{};
   c.nsmall = n;
   c.tw = tw;
-  HIPCHK(hipMalloc(&st_buf.p, sizeof(DevState)));
-  DevState* dst = (DevState*)st_buf.p;
-  HIPCHK(hipMemsetAsync(dst, 0, sizeof(DevState), st));
-  c.st = dst;
+  c.st = (DevState*)work;
   PlainRowArgs p{};
   p.nlines = lines;
   p.tw = tw;
+  int rc;
   // leading complex axes (axis d-2 .. 0), in place on the complex array
   auto leading = [&](void* arr, int dir) -> int {
     for (int ax = dim - 2; ax >= 0; --ax) {
@@ -945,7 +998,28 @@ int fft_T(int dim, int n, int howmany, bool forward, void* in, void* out, hipStr
     if ((rc = launch_plain_row<T, +1>(n, p, st))) return rc;
   }
   HIPCHK(hipGetLastError());
+  return AZB_OK;
+}
+
+// the allocating form: a temporary work area, freed after the transform finished
+template <typename T>
+int fft_T(int dim, int n, int howmany, bool forward, void* in, void* out, hipStream_t st) {
+  if (!good_len(n)) return fail(AZB_ERR_UNSUPPORTED, "FFT length " + std::to_string(n) + " not built");
+  if (dim < 1 || dim > 3) return fail(AZB_ERR_ARG, "dim must be 1..3");
+  DevFree work;
+  HIPCHK(hipMalloc(&work.p, fft_work_bytes<T>(n)));
+  int rc = fft_work_init<T>(work.p, n);
+  if (rc) return rc;
+  if ((rc = fft_ws_T<T>(dim, n, howmany, forward, in, out, work.p, st))) return rc;
   HIPCHK(hipStreamSynchronize(st));
+  return AZB_OK;
+}
+
+int fft_check(int precision, int howmany, const void* in, const void* out, int n) {
+  if (!in || !out) return fail(AZB_ERR_ARG, "NULL argument");
+  if (precision != AZB_FP64 && precision != AZB_FP32_MIXED) return fail(AZB_ERR_ARG, "unknown precision");
+  if (howmany < 1) return fail(AZB_ERR_ARG, "howmany must be positive");
+  if (!good_len(n)) return fail(AZB_ERR_UNSUPPORTED, "FFT length " + std::to_string(n) + " not built");
   return AZB_OK;
 }
 }  // namespace
@@ -953,21 +1027,50 @@ int fft_T(int dim, int n, int howmany, bool forward, void* in, void* out, hipStr
 extern "C" {
 
 int azb_fft_r2c(int dim, int n, int howmany, int precision, const void* in, void* out, void* stream) {
-  if (!in || !out) return fail(AZB_ERR_ARG, "NULL argument");
-  if (precision != AZB_FP64 && precision != AZB_FP32_MIXED) return fail(AZB_ERR_ARG, "unknown precision");
-  if (howmany < 1) return fail(AZB_ERR_ARG, "howmany must be positive");
+  int rc = fft_check(precision, howmany, in, out, n);
+  if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
   return precision == AZB_FP64 ? fft_T<double>(dim, n, howmany, true, (void*)in, out, st)
                                : fft_T<float>(dim, n, howmany, true, (void*)in, out, st);
 }
 
 int azb_fft_c2r(int dim, int n, int howmany, int precision, void* in, void* out, void* stream) {
-  if (!in || !out) return fail(AZB_ERR_ARG, "NULL argument");
-  if (precision != AZB_FP64 && precision != AZB_FP32_MIXED) return fail(AZB_ERR_ARG, "unknown precision");
-  if (howmany < 1) return fail(AZB_ERR_ARG, "howmany must be positive");
+  int rc = fft_check(precision, howmany, in, out, n);
+  if (rc) return rc;
   hipStream_t st = (hipStream_t)stream;
   return precision == AZB_FP64 ? fft_T<double>(dim, n, howmany, false, in, out, st)
                                : fft_T<float>(dim, n, howmany, false, in, out, st);
+}
+
+size_t azb_fft_work_bytes(int n, int precision) {
+  if (!good_len(n) || (precision != AZB_FP64 && precision != AZB_FP32_MIXED)) return 0;
+  return precision == AZB_FP64 ? fft_work_bytes<double>(n) : fft_work_bytes<float>(n);
+}
+
+int azb_fft_work_init(void* work, int n, int precision) {
+  if (!work) return fail(AZB_ERR_ARG, "NULL work area");
+  if (precision != AZB_FP64 && precision != AZB_FP32_MIXED) return fail(AZB_ERR_ARG, "unknown precision");
+  if (!good_len(n)) return fail(AZB_ERR_UNSUPPORTED, "FFT length " + std::to_string(n) + " not built");
+  return precision == AZB_FP64 ? fft_work_init<double>(work, n) : fft_work_init<float>(work, n);
+}
+
+int azb_fft_r2c_ws(int dim, int n, int howmany, int precision, const void* in, void* out, void* work,
+                   void* stream) {
+  int rc = fft_check(precision, howmany, in, out, n);
+  if (rc) return rc;
+  if (!work) return fail(AZB_ERR_ARG, "NULL work area");
+  hipStream_t st = (hipStream_t)stream;
+  return precision == AZB_FP64 ? fft_ws_T<double>(dim, n, howmany, true, (void*)in, out, work, st)
+                               : fft_ws_T<float>(dim, n, howmany, true, (void*)in, out, work, st);
+}
+
+int azb_fft_c2r_ws(int dim, int n, int howmany, int precision, void* in, void* out, void* work, void* stream) {
+  int rc = fft_check(precision, howmany, in, out, n);
+  if (rc) return rc;
+  if (!work) return fail(AZB_ERR_ARG, "NULL work area");
+  hipStream_t st = (hipStream_t)stream;
+  return precision == AZB_FP64 ? fft_ws_T<double>(dim, n, howmany, false, in, out, work, st)
+                               : fft_ws_T<float>(dim, n, howmany, false, in, out, work, st);
 }
 
 int azb_copy_to_padded(int dim, int N, int M, const void* in, void* out, void* stream) {

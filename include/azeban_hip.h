@@ -109,6 +109,14 @@ int azb_simulate_until(azb_solver* s, void* u_hat, double t_end, double C, doubl
  * step past t_end does nothing on the device, so extra steps are harmless);
  * azb_loop_read copies each sample's time and step count back (synchronising). */
 int azb_loop_begin(azb_solver* s, double t_end, double C, const double* times0_host);
+/* (Re)start one sample of a batched solver without touching the others and without
+ * synchronising: queued on the solver's stream, so it takes effect after the steps
+ * already queued there and before the next ones.  Sets that sample's time to time0,
+ * its target to t_end and its step count to 0.  Every sample of a solver shares one
+ * CFL number C (AZB_ERR_ARG otherwise).  This is the per-request start of the
+ * continuous-batching server: each request is one Simulation::simulate_until
+ * (src/azeban/simulation.cpp:89-102) in a slot of a batched solver. */
+int azb_loop_begin_sample(azb_solver* s, int sample, double t_end, double C, double time0);
 int azb_enqueue_steps(azb_solver* s, void* u_hat, int nsteps);
 int azb_loop_read(azb_solver* s, double* times_host, int* steps_host);
 /* Non-blocking variant: azb_loop_probe queues a copy of every sample's time and
@@ -127,6 +135,10 @@ int azb_graph_destroy(azb_graph* g);
 int azb_stream_create(void** stream);
 int azb_stream_destroy(void* stream);
 int azb_stream_synchronize(void* stream);
+/* Work queued on `waiter` after this call starts only once everything queued on
+ * `waitee` so far has finished (an event record + wait; inside a capture this joins
+ * the two streams' branches of the graph).  Does not block the host. */
+int azb_stream_wait(void* waiter, void* waitee);
 
 /* ---- per-kernel-family device timing (hipEvent pairs on the solver's stream),
  * the analogue of azeban's ProfileDevice records (include/azeban/profiler.hpp:35-135). */
@@ -167,6 +179,20 @@ int azb_fft_r2c(int dim, int n, int howmany, int precision, const void* in, void
  * (fftwfft.cpp:265-269, cufft.cpp:143-149). */
 int azb_fft_c2r(int dim, int n, int howmany, int precision, void* in, void* out, void* stream);
 
+/* The same transforms from a caller-owned work area, for FFT<Dim>::set_work_area /
+ * get_work_area_size (include/azeban/operations/fft.hpp:88-90,
+ * src/azeban/cuda/operations/cufft.cpp:129-140,251-262).  The work area of a length-n
+ * transform holds its twiddle table: azb_fft_work_bytes gives its size (0 for a length
+ * or precision that is not built), azb_fft_work_init fills it (synchronising), and
+ * transforms of the same n and precision may share one.  The _ws transforms allocate
+ * nothing and do not synchronise (queued on `stream`). */
+size_t azb_fft_work_bytes(int n, int precision);
+int azb_fft_work_init(void* work, int n, int precision);
+int azb_fft_r2c_ws(int dim, int n, int howmany, int precision, const void* in, void* out, void* work,
+                   void* stream);
+int azb_fft_c2r_ws(int dim, int n, int howmany, int precision, void* in, void* out, void* work,
+                   void* stream);
+
 /* One component N -> M zero padding, copy_to_padded
  * (src/azeban/operations/copy_to_padded.cpp:301-328,
  *  include/azeban/operations/copy_to_padded_functions.hpp:11-63). */
@@ -204,6 +230,7 @@ int azb_malloc(void** ptr, size_t bytes);
 int azb_free(void* ptr);
 int azb_memcpy_h2d(void* dst, const void* src_host, size_t bytes);
 int azb_memcpy_d2h(void* dst_host, const void* src, size_t bytes);
+int azb_memcpy_d2d(void* dst, const void* src, size_t bytes);
 int azb_memcpy_h2d_async(void* dst, const void* src_host, size_t bytes, void* stream);
 int azb_memcpy_d2h_async(void* dst_host, const void* src, size_t bytes, void* stream);
 int azb_synchronize(azb_solver* s);

@@ -29,7 +29,10 @@ Parity status per function (what pins it; see tests/test_oracle.py):
   ssp_rk3 / simulate      pinned: linear-ODE closed form (RK3 stability polynomial),
                           dt sequence and last-step clamp
   leray                   pinned: divergence at round-off, idempotence
-  energy_spectrum         pinned: Parseval identity (sum E(K) = 1/2 <|u|^2>)
+  energy_spectrum         pinned: single-mode known answers (L-infinity shell index
+                          and conjugate doubling at k_last = 0, interior, N/2, which
+                          an L2-shell or no-Nyquist-doubling rule fails), physical-
+                          field energies, Parseval identity
 """
 from __future__ import annotations
 

@@ -1,11 +1,15 @@
 // C++ drop-in layer tests (include/azeban_hip.hpp), written in the style of the
 // reference's Catch2 tests: FFT vs brute-force DFT (test/azeban/operations/fft.cpp:
-// 35-59,219-280), Taylor-Green steady state, the SSP-RK3 linear closed form
-// (tests the same property as test/azeban/equations/spectral_viscosity.cpp:49-104,
-// tracer-free), the simulate_until clamp and the Leray invariant.
+// 35-59,219-280) through make_fft with an owned and a shared work area, Taylor-Green
+// steady state, the SSP-RK3 linear closed form (tests the same property as
+// test/azeban/equations/spectral_viscosity.cpp:49-104, tracer-free) driven through
+// TimeIntegrator<Dim>* / Equation<Dim>* base pointers, the reference's generic RK3
+// sequence for a user-defined Equation against the fused step, the simulate_until
+// clamp and both Simulation constructors.
 #include <azeban_hip.hpp>
 
 #include <cmath>
+#include <complex>
 #include <cstdio>
 #include <random>
 #include <vector>
@@ -59,8 +63,11 @@ static void test_fft(int n, int nvars) {
   std::vector<double> h(u.size());
   for (auto& v : h) v = dist(rng);
   u.copy_to_device(h);
-  azeban::FFT<Dim> fft(uh, u);
-  fft.forward();
+  // FFT<Dim>* from the factory, owning its work area (fft_factory.hpp:27-31)
+  std::shared_ptr<azeban::FFT<Dim>> fft = azeban::make_fft<Dim>(uh.view(), u.view());
+  REQUIRE(fft->is_forward() && fft->is_backward());
+  REQUIRE(fft->shape_u_hat(u.shape()) == uh.shape() && fft->shape_u(uh.shape()) == u.shape());
+  fft->forward();
   auto hh = uh.copy_to_host();
   long per = h.size() / nvars, perc = hh.size() / nvars;
   double err = 0;
@@ -75,7 +82,14 @@ static void test_fft(int n, int nvars) {
     }
   }
   REQUIRE(err <= 1e-8);
-  fft.backward();
+  // a second transform planned without a work area, given a shared one
+  // (FFT::initialize(..., false) + get_work_area_size / set_work_area, fft.hpp:86-90)
+  auto fft2 = azeban::make_fft<Dim, real_t>(azeban::device_type::hip, azeban::FFT_BACKWARD);
+  fft2->initialize(uh.view(), u.view(), false);
+  REQUIRE(!fft2->is_forward() && fft2->is_backward());
+  REQUIRE(fft2->get_work_area_size() == fft->get_work_area_size());
+  fft2->set_work_area(fft->get_work_area());
+  fft2->backward();
   auto back = u.copy_to_host();
   double nd = std::pow(double(n), Dim), err2 = 0;
   for (size_t i = 0; i < h.size(); ++i) err2 = std::max(err2, std::abs(back[i] / nd - h[i]));
@@ -88,7 +102,7 @@ static azeban::device_array<complex_t, 3> init2d(int N, const std::vector<double
   azeban::device_array<real_t, 3> u({2, N, N});
   u.copy_to_device(phys);
   azeban::device_array<complex_t, 3> uh({2, N, N / 2 + 1});
-  azeban::FFT<2>(uh, u).forward();
+  azeban::make_fft<2>(uh.view(), u.view(), azeban::FFT_FORWARD)->forward();
   azeban::leray<2>(uh);
   return uh;
 }
@@ -126,21 +140,23 @@ static void test_rk3_linear_closed_form() {
   auto uh0 = init2d(N, p);
   azeban::Grid<2> grid(N);
   azeban::SmoothCutoff1D visc(0.05 / N, 1, 2 * pi * N / 4);
-  auto eq = std::make_shared<azeban::IncompressibleEuler<2>>(grid, visc);
-  auto rk = std::make_shared<azeban::SSP_RK3<2>>(azeban::device_type::hip, grid.shape_fourier(2), eq);
-  azeban::Simulation<2> sim(grid, 0.2, rk, azeban::device_type::hip);
+  // the reference's construction chain, held through the base classes
+  std::shared_ptr<azeban::Equation<2>> eq = std::make_shared<azeban::IncompressibleEuler<2>>(grid, visc);
+  std::shared_ptr<azeban::TimeIntegrator<2>> rk
+      = std::make_shared<azeban::SSP_RK3<2>>(azeban::device_type::hip, grid.shape_fourier(2), eq);
+  azeban::Simulation<2> sim(uh0.view(), grid, 0.2, rk);
+  REQUIRE(sim.equation() == eq && rk->equation() == eq && sim.memory_location() == azeban::device_type::hip);
   auto h0 = uh0.copy_to_host();
-  sim.u().copy_to_device(h0);
   const double nu = visc.eval(2 * pi * m);
   double ratio = 1.0, t = 0;
   for (int s = 0; s < 5; ++s) {
-    const double max_dt = std::min(1.0 - t, 0.2 * 2. / (visc.eps * (pi * N) * (pi * N)));
-    const double dt = rk->integrate(1.0, max_dt, 0.2, sim.u().view());
+    const double max_dt = std::min(1.0 - t, 0.2 * 2. / (eq->visc() * (pi * N) * (pi * N)));
+    const double dt = rk->integrate(1.0, max_dt, 0.2, sim.u());
     const double z = nu * dt;
     ratio *= 1 + z + z * z / 2 + z * z * z / 6;
     t += dt;
   }
-  auto h = sim.u().copy_to_host();
+  auto h = azeban::device_array<complex_t, 3>::to_host(sim.u());
   const long idx = m;  // component 0, row 0, column m
   const double expect_x = h0[idx].x * ratio, expect_y = h0[idx].y * ratio;
   // sin gives a purely imaginary mode: compare against the mode's magnitude
@@ -148,6 +164,63 @@ static void test_rk3_linear_closed_form() {
   REQUIRE(std::abs(h[idx].x - expect_x) <= 1e-12 * mag);
   REQUIRE(std::abs(h[idx].y - expect_y) <= 1e-12 * mag);
   std::printf("RK3 linear closed form: ratio %.15f\n", ratio);
+}
+
+// A user-defined Equation (not IncompressibleEuler) takes SSP_RK3's generic path: the
+// reference's three dudt calls and five axpbys (ssp_rk3.hpp:48-64) on device arrays.
+// Wrapping the MI355X equation, it must reproduce the fused step to round-off.
+template <int Dim>
+class Wrapped final : public azeban::Equation<Dim> {
+ public:
+  explicit Wrapped(std::shared_ptr<azeban::Equation<Dim>> inner) : azeban::Equation<Dim>(Grid0()), in_(inner) {}
+  void dudt(const azeban::array_view<complex_t, Dim + 1>& d, const azeban::array_const_view<complex_t, Dim + 1>& u,
+            double t, double dt, double C) override {
+    ++calls;
+    in_->dudt(d, u, t, dt, C);
+  }
+  double dt(double C) const override { return in_->dt(C); }
+  int n_vars() const override { return in_->n_vars(); }
+  real_t visc() const override { return in_->visc(); }
+  int calls = 0;
+
+ private:
+  static azeban::Grid<Dim> Grid0() { return azeban::Grid<Dim>(); }
+  std::shared_ptr<azeban::Equation<Dim>> in_;
+};
+
+static void test_generic_rk3_matches_fused() {
+  const int N = 64;
+  std::vector<double> p(2 * N * N);
+  for (int i = 0; i < N; ++i)
+    for (int j = 0; j < N; ++j) {
+      const double y = double(j) / N, x = double(i) / N;
+      p[i * N + j] = y <= 0.5 ? std::tanh((y - 0.25) / 0.05) : std::tanh((0.75 - y) / 0.05);
+      p[N * N + i * N + j] = 0.05 * std::sin(2 * pi * x);
+    }
+  auto uh = init2d(N, p);
+  azeban::Grid<2> grid(N);
+  azeban::SmoothCutoff1D visc(0.05 / N, 1, 2 * pi * N / 4);
+  auto fused_eq = std::make_shared<azeban::IncompressibleEuler<2>>(grid, visc);
+  std::shared_ptr<azeban::TimeIntegrator<2>> fused
+      = std::make_shared<azeban::SSP_RK3<2>>(azeban::device_type::hip, grid.shape_fourier(2), fused_eq);
+  auto wrapped = std::make_shared<Wrapped<2>>(std::make_shared<azeban::IncompressibleEuler<2>>(grid, visc));
+  std::shared_ptr<azeban::TimeIntegrator<2>> generic
+      = std::make_shared<azeban::SSP_RK3<2>>(azeban::device_type::hip, grid.shape_fourier(2), wrapped);
+  azeban::Simulation<2> a(uh.view(), grid, 0.2, fused), b(uh.view(), grid, 0.2, generic);
+  a.simulate_until(0.05);
+  b.simulate_until(0.05);
+  REQUIRE(a.time() == 0.05 && b.time() == 0.05);
+  REQUIRE(a.steps() == b.steps() && wrapped->calls == 3 * b.steps());
+  auto ha = azeban::device_array<complex_t, 3>::to_host(a.u());
+  auto hb = azeban::device_array<complex_t, 3>::to_host(b.u());
+  double num = 0, den = 0;
+  for (size_t i = 0; i < ha.size(); ++i) {
+    num += std::pow(ha[i].x - hb[i].x, 2) + std::pow(ha[i].y - hb[i].y, 2);
+    den += ha[i].x * ha[i].x + ha[i].y * ha[i].y;
+  }
+  REQUIRE(std::sqrt(num / den) <= 1e-13);
+  std::printf("generic SSP_RK3 (dudt + axpby) vs fused step: %ld steps, rel L2 %.1e\n", b.steps(),
+              std::sqrt(num / den));
 }
 
 static void test_simulate_until_clamp() {
@@ -165,24 +238,25 @@ static void test_simulate_until_clamp() {
   auto eq = std::make_shared<azeban::IncompressibleEuler<2>>(grid, visc);
   auto rk = std::make_shared<azeban::SSP_RK3<2>>(azeban::device_type::hip, grid.shape_fourier(2), eq);
   azeban::Simulation<2> a(grid, 0.2, rk, azeban::device_type::hip);
-  a.u().copy_to_device(uh.copy_to_host());
+  azeban::copy<complex_t, 3>(a.u(), uh.view());
   a.simulate_until(0.05);
   auto eq2 = std::make_shared<azeban::IncompressibleEuler<2>>(grid, visc);
   auto rk2 = std::make_shared<azeban::SSP_RK3<2>>(azeban::device_type::hip, grid.shape_fourier(2), eq2);
   azeban::Simulation<2> b(grid, 0.2, rk2, azeban::device_type::hip);
-  b.u().copy_to_device(uh.copy_to_host());
+  azeban::copy<complex_t, 3>(b.u(), uh.view());
   b.simulate_until_device(0.05, 4);
   REQUIRE(a.time() == 0.05);
   REQUIRE(b.time() == 0.05);
   REQUIRE(a.steps() == b.steps());
-  auto ha = a.u().copy_to_host(), hb = b.u().copy_to_host();
+  auto ha = azeban::device_array<complex_t, 3>::to_host(a.u());
+  auto hb = azeban::device_array<complex_t, 3>::to_host(b.u());
   double num = 0, den = 0;
   for (size_t i = 0; i < ha.size(); ++i) {
     num += std::pow(ha[i].x - hb[i].x, 2) + std::pow(ha[i].y - hb[i].y, 2);
     den += ha[i].x * ha[i].x + ha[i].y * ha[i].y;
   }
   REQUIRE(std::sqrt(num / den) <= 1e-14);
-  auto e = azeban::energy_spectrum<2>(grid, a.u().view());
+  auto e = azeban::energy_spectrum<2>(grid, a.u());
   REQUIRE((long)e.size() == grid.N_fourier);
   std::printf("simulate_until: %ld host-loop steps, device loop agrees to %.1e\n", a.steps(), std::sqrt(num / den));
 }
@@ -196,6 +270,7 @@ int main() {
   test_fft<3>(32, 2);
   test_taylor_green_steady();
   test_rk3_linear_closed_form();
+  test_generic_rk3_matches_fused();
   test_simulate_until_clamp();
   std::printf("%d checks, %d failed\n", g_checks, g_fail);
   return g_fail == 0 ? 0 : 1;

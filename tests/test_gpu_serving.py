@@ -65,3 +65,73 @@ def test_host_side_requests_upload_asynchronously(cuda):
     for res in b.results:
         assert res.steps == ra[res.rid].steps
         assert torch.equal(res.u_hat, ra[res.rid].u_hat), res.rid
+
+
+def test_slots_are_reused_by_later_requests(cuda):
+    """More requests of one size than its class has slots: later requests join the
+    running batch in slots that earlier ones left (a slot's stale progress from its
+    previous request never ends the new one early); each result equals a solve alone."""
+    print("[case] serving slot reuse", flush=True)
+    rs = []
+    for i in range(7):
+        u = initialize(I.fractional_brownian(2, 64, seed=300 + i))
+        rs.append(Request(i, 2, 64, u, 0.004 + 0.003 * (i % 3), 0.0005 * i))
+    torch.cuda.synchronize()
+    srv = Server(chunk_steps=2, slots={(2, 64): 2})
+    stats = srv.run(rs)
+    assert stats["requests"] == 7 and stats["peak_active"] == 2
+    by_id = {r.rid: r for r in rs}
+    for res in srv.results:
+        r = by_id[res.rid]
+        ref, n = solve_one(r.dim, r.N, r.u_hat, r.t_end)
+        assert res.steps == n, r.rid
+        assert torch.equal(res.u_hat, ref), r.rid
+
+
+def test_serving_config_sizes_against_single_solves_and_oracle(cuda):
+    """The serving workload's larger sizes (2D 512^2, 1024^2; 3D 128^3, 256^3, fractional
+    Brownian data) through the server in one mix: bit-identical to solving each alone,
+    and the 1024^2 and 128^3 requests match the FP64 oracle to 1e-11."""
+    import numpy as np
+
+    from oracle import azeban_oracle as O
+
+    print("[case] serving config sizes", flush=True)
+    specs = [(2, 512, 0.004), (2, 1024, 0.002), (3, 128, 0.004), (3, 256, 0.002), (2, 1024, 0.003)]
+    rs = []
+    for i, (dim, N, t_end) in enumerate(specs):
+        u = initialize(I.fractional_brownian(dim, N, seed=500 + i))
+        rs.append(Request(i, dim, N, u, t_end, 0.001 * i))
+    torch.cuda.synchronize()
+    srv = Server(chunk_steps=4)
+    stats = srv.run(rs)
+    assert stats["requests"] == len(rs) and stats["peak_active"] >= 2
+    by_id = {r.rid: r for r in rs}
+    for res in srv.results:
+        r = by_id[res.rid]
+        ref, n = solve_one(r.dim, r.N, r.u_hat, r.t_end)
+        assert res.steps == n, r.rid
+        assert torch.equal(res.u_hat, ref), r.rid
+        if (r.dim, r.N) in ((2, 1024), (3, 128)) and r.rid != 4:
+            osim = O.Simulation(O.IncompressibleEuler(O.Grid(r.dim, r.N), O.baseline_visc(r.N), workers=16),
+                                r.u_hat.cpu().numpy(), 0.2)
+            osim.simulate_until(r.t_end)
+            err = O.l2_rel(res.u_hat.cpu().numpy(), osim.u)
+            print(f"  {r.dim}D N={r.N} steps={n} rel L2 vs oracle {err:.2e}", flush=True)
+            assert len(osim.dts) == n and err <= 1e-11, (r.rid, err)
+
+
+def test_one_at_a_time_baseline_gives_the_same_results(cuda):
+    """The sequential baseline solves the same requests to the same results."""
+    from azeban_amd.serving import serve_one_at_a_time
+
+    print("[case] serving one-at-a-time baseline", flush=True)
+    rs = reqs()
+    srv = Server(chunk_steps=3)
+    srv.run(rs)
+    base = serve_one_at_a_time(rs, chunk=3)
+    assert base["requests"] == len(rs) and base["latency_p99_s"] >= base["latency_p50_s"] > 0
+    got = {r.rid: r for r in base["results"]}
+    for res in srv.results:
+        assert res.steps == got[res.rid].steps
+        assert torch.equal(res.u_hat, got[res.rid].u_hat), res.rid
