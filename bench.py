@@ -114,34 +114,126 @@ def run_precision(A, dim, N, u_hat0, precision, steps, warmup):
     return {"eq": eq, "sim": sim, "wall_s": wall, "gpu_ms": gpu_ms, "families": fam, "tuning": tuning}
 
 
-def cpu_baseline(dim: int, budget_s: float):
-    """The FP64 numpy oracle (pocketfft, one thread) on a bounded sample."""
-    import numpy as np
+def cpu_info() -> dict:
+    """The host the CPU baseline runs on: CPU model, usable cores, and whether a system
+    FFTW library is present (the reference's CPU FFT)."""
+    import subprocess
 
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    try:
+        res = subprocess.run(["ldconfig", "-p"], capture_output=True, text=True, timeout=30)
+        fftw = sorted({ln.split()[0] for ln in res.stdout.splitlines() if "fftw3" in ln})
+    except (OSError, subprocess.SubprocessError):
+        fftw = []
+    return {"cpu_model": model, "cores": cores, "fftw3_libraries": fftw or "none (ldconfig -p lists no fftw3)"}
+
+
+def _oracle_setup(dim: int, Ns: int):
     sys.path.insert(0, os.path.dirname(os.path.abspath(__file__)))
     from oracle import azeban_oracle as O
     from azeban_amd import init as I
 
+    if dim == 3:
+        u0 = O.initialize(I.taylor_green_3d(Ns))
+    elif Ns <= 128:
+        u0 = O.initialize(I.taylor_green_2d(Ns))
+    else:
+        u0 = O.initialize(I.double_shear_layer(Ns))
+    return O, u0
+
+
+def time_oracle(dim: int, Ns: int, workers, threads: int, steps: int = 0, t_end: float = 0.0, runs: int = 3,
+                u0=None):
+    """Median over `runs` of the oracle's seconds per SSP-RK3 step (either `steps` steps or
+    to `t_end`), with FFTs on `workers` threads (None: numpy's single-threaded pocketfft)
+    and compute_B / compute_dudt on `threads` threads.  Returns (record, last simulation)."""
+    O, u_init = _oracle_setup(dim, Ns)
+    u0 = u_init if u0 is None else u0
+    per_step, sim = [], None
+    for run in range(runs):
+        eq = O.IncompressibleEuler(O.Grid(dim, Ns), O.baseline_visc(Ns), workers=workers, threads=threads)
+        sim = O.Simulation(eq, u0, 0.2)
+        t0 = time.perf_counter()
+        if t_end > 0:
+            sim.simulate_until(t_end)
+            n = len(sim.dts)
+        else:
+            for _ in range(steps):
+                sim.step(1e9)
+            n = steps
+        el = time.perf_counter() - t0
+        per_step.append(el / n)
+        log(f"cpu {dim}D N={Ns} workers={workers} threads={threads} run {run}: {n} steps, {el:.2f} s")
+    med = sorted(per_step)[len(per_step) // 2]
+    return {"s_per_step": med, "Mpts_RK_steps_per_s": Ns**dim / med / 1e6, "steps_per_run": n, "runs": runs,
+            "fft_threads": workers or 1, "loop_threads": threads}, sim
+
+
+def cpu_baseline(dim: int, runs: int = 3):
+    """The FP64 oracle on a bounded sample of the workload (3D Taylor-Green 128^3 or the 2D
+    double shear layer 1024^2), median of `runs` runs, in three threadings: the
+    reference's CPU path (single-threaded FFT, compute_B / compute_dudt on every core, as
+    its OpenMP loops), one thread, and every stage on every core.  The headline value is
+    the reference threading."""
+    info = cpu_info()
+    ncores = info["cores"]
     Ns = 128 if dim == 3 else 1024
-    u0 = O.initialize(I.taylor_green_3d(Ns) if dim == 3 else I.double_shear_layer(Ns))
-    eq = O.IncompressibleEuler(O.Grid(dim, Ns), O.baseline_visc(Ns))
-    sim = O.Simulation(eq, u0, 0.2)
-    t0 = time.perf_counter()
-    steps = 0
-    while time.perf_counter() - t0 < budget_s or steps == 0:
-        sim.step(1e9)
-        steps += 1
-    el = time.perf_counter() - t0
+    steps = 2 if dim == 3 else 5
+    ref, sim = time_oracle(dim, Ns, None, ncores, steps=steps, runs=runs)
+    one, _ = time_oracle(dim, Ns, None, 1, steps=steps, runs=runs)
+    allc, _ = time_oracle(dim, Ns, ncores, ncores, steps=steps, runs=runs)
     name = f"3D Taylor-Green {Ns}^3" if dim == 3 else f"2D double shear layer {Ns}^2"
+    O, u0 = _oracle_setup(dim, Ns)
     return {
-        "value": Ns**dim * steps / el / 1e6,
+        "value": ref["Mpts_RK_steps_per_s"],
         "unit": "Mpts*RK-steps/s",
-        "cores": 1,
+        "cores": ncores,
         "kind": "port",
-        "sample": f"{name}, {steps} SSP-RK3 steps in {el:.1f} s, FP64 numpy oracle "
-                  f"(pocketfft, not FFTW), 1 thread",
+        "sample": (f"{name}, {steps} SSP-RK3 steps per run, median of {runs} runs, FP64 numpy oracle with "
+                   f"the reference's CPU threading: FFT single-threaded (numpy pocketfft, not FFTW), "
+                   f"compute_B / compute_dudt on {ncores} threads"),
+        "host": info,
+        "reference_threading": ref,
+        "one_thread": one,
+        "all_threads": allc,
         "gpu_fp64_vs_oracle": _gpu_vs_oracle(dim, Ns, u0, sim, steps),
     }
+
+
+def run_cpu_plan(args):
+    """The CPU-baseline plan in full, on this host: 2D Taylor-Green 128^2 to t = 1 and the
+    2D double shear layer 1024^2 for 100 steps with the reference's threading (and the
+    former on one thread), and one 3D Taylor-Green 512^3 step per run with every stage on
+    every core (a single-threaded 768^3 FFT takes minutes per transform).  Median of
+    `--cpu-runs` runs each."""
+    info = cpu_info()
+    nc = info["cores"]
+    runs = args.cpu_runs
+    out = {"metric": "CPU baseline (FP64 oracle)", "unit": "Mpts*RK-steps/s", "host": info, "cases": {}}
+    log("cpu plan: 2D TG 128^2 to t=1")
+    out["cases"]["tg2d_128_t1_reference_threading"], _ = time_oracle(2, 128, None, nc, t_end=1.0, runs=runs)
+    out["cases"]["tg2d_128_t1_one_thread"], _ = time_oracle(2, 128, None, 1, t_end=1.0, runs=runs)
+    log("cpu plan: 2D DSL 1024^2, 100 steps")
+    out["cases"]["dsl2d_1024_100_steps_reference_threading"], _ = time_oracle(2, 1024, None, nc, steps=100,
+                                                                              runs=runs)
+    log("cpu plan: 3D TG 512^3")
+    out["cases"]["tg3d_512_all_threads"], _ = time_oracle(3, 512, nc, nc, steps=args.cpu_512_steps, runs=runs)
+    out["note"] = ("FFT: numpy/scipy pocketfft, not FFTW (no fftw3 library on the host); 'reference "
+                   "threading' = single-threaded FFT with compute_B / compute_dudt on every core, as "
+                   "the reference's CPU path; the 512^3 case runs its FFTs on every core too")
+    print(json.dumps(out), flush=True)
 
 
 def _gpu_vs_oracle(dim, Ns, u0, osim, steps):
@@ -246,28 +338,61 @@ def run_ensemble(args):
     print(json.dumps(out), flush=True)
 
 
-def run_serving(args):
-    """Poisson arrivals (mean rate 50/s) of 2D 128^2-1024^2 and 3D 64^3-256^3 solves of
-    fractional-Brownian data (H = 0.75) to t = 0.5, continuous batching by HBM footprint."""
+def _serve_pair(reqs, args, label):
+    """The request set through the continuous-batching server and one at a time; both
+    runs see the same arrival times."""
     import torch
 
-    from azeban_amd.serving import Server, poisson_requests
+    from azeban_amd.serving import Server, serve_one_at_a_time
 
-    t_end = args.t_end if args.t_end > 0 else 0.5
-    log(f"serving: generating {args.requests} requests")
-    reqs = poisson_requests(args.requests, 50.0, seed=1, t_end=t_end)
+    log(f"serving {label}: server")
     srv = Server(hbm_budget=int(args.hbm_gib * 2**30), max_active=args.max_active, chunk_steps=args.ens_chunk)
-    log("serving: running")
     st = srv.run(reqs)
     torch.cuda.synchronize()
+    log(f"serving {label}: one at a time")
+    base = serve_one_at_a_time(reqs, chunk=args.ens_chunk)
+    got = {r.rid: r for r in base.pop("results")}
+    identical = all(res.steps == got[res.rid].steps and bool(torch.equal(res.u_hat, got[res.rid].u_hat))
+                    for res in srv.results)
+
+    def rec(x):
+        return {"requests_per_s": round(x["throughput_per_s"], 4), "latency_p50_s": round(x["latency_p50_s"], 4),
+                "latency_p99_s": round(x["latency_p99_s"], 4), "wall_s": round(x["wall_s"], 3),
+                "Mpts_RK_steps_per_s": round(x["grid_point_steps"] / x["wall_s"] / 1e6, 2)}
+
+    server = rec(st)
+    server.update({"peak_active": st["peak_active"], "chunk_launches": st["chunk_launches"],
+                   "peak_solver_GiB": round(st["peak_solver_bytes"] / 2**30, 2)})
+    return {"server": server, "one_at_a_time": rec(base),
+            "throughput_vs_one_at_a_time": round(st["throughput_per_s"] / base["throughput_per_s"], 3),
+            "identical_results": identical}
+
+
+def run_serving(args):
+    """Poisson arrivals of 2D 128^2-1024^2 and 3D 64^3-256^3 solves of fractional-Brownian
+    data (H = 0.75) to t = 0.5 through the continuous-batching server and, on the same
+    arrivals, one request at a time: at the configured mean rate of 50/s (far above what
+    one GPU serves, so latencies are mostly queueing) and at a rate the server sustains
+    (latencies then show service time)."""
+    from azeban_amd.serving import poisson_requests
+
+    t_end = args.t_end if args.t_end > 0 else 0.5
+    log(f"serving: generating {args.requests} requests at {args.rate}/s")
+    reqs = poisson_requests(args.requests, args.rate, seed=1, t_end=t_end)
+    hot = _serve_pair(reqs, args, f"{args.rate}/s")
+    del reqs
     out = {"metric": METRIC, "unit": "requests/s", "higher_is_better": True, "n_gpus": 1, "dtype": "f64",
-           "data": "synthetic", "value": round(st["throughput_per_s"], 3),
-           "latency_p50_s": round(st["latency_p50_s"], 4), "latency_p99_s": round(st["latency_p99_s"], 4),
-           "wall_s": round(st["wall_s"], 3), "peak_active": st["peak_active"],
-           "Mpts_RK_steps_per_s": round(st["grid_point_steps"] / st["wall_s"] / 1e6, 2),
-           "config": {"workload": "serving: Poisson arrivals mean 50/s, 2D 128^2-1024^2 and 3D 64^3-256^3, "
-                                  f"fBm H=0.75, t={t_end}, FP64", "requests": args.requests,
-                      "hbm_budget_GiB": args.hbm_gib, "max_active": args.max_active}}
+           "data": "synthetic", "value": hot["server"]["requests_per_s"],
+           "latency_p50_s": hot["server"]["latency_p50_s"], "latency_p99_s": hot["server"]["latency_p99_s"],
+           "config": {"workload": f"serving: Poisson arrivals mean {args.rate}/s, 2D 128^2-1024^2 and "
+                                  f"3D 64^3-256^3, fBm H=0.75, t={t_end}, FP64", "requests": args.requests,
+                      "hbm_budget_GiB": args.hbm_gib, "max_active": args.max_active,
+                      "chunk_steps": args.ens_chunk},
+           f"rate_{args.rate:g}_per_s": hot}
+    if args.sustain_requests > 0:
+        log(f"serving: {args.sustain_requests} requests at {args.sustain_rate}/s")
+        reqs = poisson_requests(args.sustain_requests, args.sustain_rate, seed=2, t_end=t_end)
+        out[f"rate_{args.sustain_rate:g}_per_s"] = _serve_pair(reqs, args, f"{args.sustain_rate}/s")
     print(json.dumps(out), flush=True)
 
 
@@ -276,19 +401,24 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", choices=["tg3d", "dsl2d", "ensemble", "serving"], default="tg3d")
+    ap.add_argument("--workload", choices=["tg3d", "dsl2d", "ensemble", "serving", "cpu"], default="tg3d")
     ap.add_argument("--t-end", type=float, default=0.0)
     ap.add_argument("--ens-chunk", type=int, default=16)
     ap.add_argument("--ens-subset", type=int, default=8)
     ap.add_argument("--requests", type=int, default=40)
+    ap.add_argument("--rate", type=float, default=50.0, help="mean Poisson arrival rate of the serving run")
+    ap.add_argument("--sustain-requests", type=int, default=16)
+    ap.add_argument("--sustain-rate", type=float, default=0.4,
+                    help="a rate the server sustains, for latencies without queueing")
     ap.add_argument("--hbm-gib", type=float, default=200.0)
-    ap.add_argument("--max-active", type=int, default=32)
+    ap.add_argument("--max-active", type=int, default=64)
     ap.add_argument("--N", type=int, default=0)
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--error-steps", type=int, default=200,
                     help="steps after which the FP32-mixed state is compared with FP64 (both continued)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cpu-runs", type=int, default=3, help="runs per CPU-baseline case (median)")
+    ap.add_argument("--cpu-512-steps", type=int, default=1, help="3D 512^3 steps per run of the CPU plan")
     ap.add_argument("--calib-copy-gib", type=float, default=0.0,
                     help="run one device-to-device copy of this size first (counter calibration)")
     ap.add_argument("--no-autotune", action="store_true")
@@ -296,6 +426,8 @@ def main():
     global AUTOTUNE
     AUTOTUNE = not args.no_autotune
 
+    if args.workload == "cpu":
+        return run_cpu_plan(args)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported "
               "(the multi-GPU sharding of this path is a design, not a build)", file=sys.stderr)
@@ -405,7 +537,7 @@ def main():
                              "kernels": kernm, "tuning": res["fp32_mixed"]["tuning"]}
     if not args.no_cpu_baseline:
         log("cpu baseline")
-        out["cpu_baseline"] = cpu_baseline(dim, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(dim, args.cpu_runs)
     print(json.dumps(out), flush=True)
 
 

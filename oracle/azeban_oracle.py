@@ -235,12 +235,28 @@ def _pad_index(i, N, M):
 class IncompressibleEuler:
     """IncompressibleEuler<Dim, SmoothCutoff1D, NoForcing> (incompressible_euler.hpp:39-156)."""
 
-    def __init__(self, grid: Grid, visc: SmoothCutoff1D, workers: int | None = None):
+    def __init__(self, grid: Grid, visc: SmoothCutoff1D, workers: int | None = None, threads: int = 1):
         self.grid = grid
         self.visc = visc
         self.workers = workers  # FFT threads (None: numpy's single-threaded pocketfft)
+        # threads of compute_B / compute_dudt, which the reference runs as OpenMP loops
+        # (incompressible_euler.hpp:176,193,236,290) while its FFTW stays single-threaded;
+        # chunks of rows along the first axis, identical results for any count
+        self.threads = max(1, int(threads))
         self.u_max = 0.0
         self.dim = grid.dim
+
+    def _rows(self, n, fn):
+        """fn(rows) over row chunks of the first spatial axis, on self.threads threads
+        (numpy releases the interpreter lock inside its element-wise loops)."""
+        if self.threads == 1:
+            return [fn(slice(0, n))]
+        from concurrent.futures import ThreadPoolExecutor
+
+        step = -(-n // self.threads)
+        chunks = [slice(a, min(n, a + step)) for a in range(0, n, step)]
+        with ThreadPoolExecutor(self.threads) as ex:
+            return list(ex.map(fn, chunks))
 
     def n_vars(self):
         return self.dim
@@ -267,14 +283,23 @@ class IncompressibleEuler:
         """incompressible_euler.hpp:168-212 with incompressible_euler_functions.hpp:26-45."""
         g = self.grid
         norm = 1.0 / (math.sqrt(g.N) ** self.dim * math.sqrt(g.M) ** self.dim)
-        if self.dim == 2:
-            u1 = norm * u[0]
-            u2 = norm * u[1]
-            return np.stack([u1 * u1, u1 * u2, u2 * u2])
-        u1 = norm * u[0]
-        u2 = norm * u[1]
-        u3 = norm * u[2]
-        return np.stack([u1 * u1, u2 * u1, u2 * u2, u3 * u1, u3 * u2, u3 * u3])
+        nB = 3 if self.dim == 2 else 6
+        B = np.empty((nB,) + u.shape[1:], dtype=u.dtype)
+
+        def rows(r):
+            if self.dim == 2:
+                u1 = norm * u[0, r]
+                u2 = norm * u[1, r]
+                B[0, r], B[1, r], B[2, r] = u1 * u1, u1 * u2, u2 * u2
+                return
+            u1 = norm * u[0, r]
+            u2 = norm * u[1, r]
+            u3 = norm * u[2, r]
+            B[0, r], B[1, r], B[2, r] = u1 * u1, u2 * u1, u2 * u2
+            B[3, r], B[4, r], B[5, r] = u3 * u1, u3 * u2, u3 * u3
+
+        self._rows(u.shape[1], rows)
+        return B
 
     def wavenumbers(self):
         """Signed wavenumbers k = 2 pi i_ of the N-grid modes and the B_hat row indices."""
@@ -288,12 +313,24 @@ class IncompressibleEuler:
     def compute_dudt(self, B_hat: np.ndarray, u_hat: np.ndarray) -> np.ndarray:
         """computeDudt_cpu_{2d,3d} (incompressible_euler.hpp:231-347) with
         incompressible_euler_{2d,3d}_compute_L (incompressible_euler_functions.hpp:47-112)."""
+        out = np.empty_like(u_hat)
+
+        def rows(r):
+            out[:, r] = self._compute_dudt_rows(B_hat, u_hat, r)
+
+        self._rows(u_hat.shape[1], rows)
+        return out
+
+    def _compute_dudt_rows(self, B_hat: np.ndarray, u_hat_all: np.ndarray, r: slice) -> np.ndarray:
+        """compute_dudt for the modes whose first index lies in r."""
         g = self.grid
         k_full, iB, k_last = self.wavenumbers()
+        k_rows, iB_rows = k_full[r], iB[r]
+        u_hat = u_hat_all[:, r]
         with np.errstate(divide="ignore", invalid="ignore"):
             if self.dim == 2:
-                Bh = B_hat[:, iB, : g.Nf]
-                k1 = k_full[:, None]
+                Bh = B_hat[:, iB_rows, : g.Nf]
+                k1 = k_rows[:, None]
                 k2 = k_last[None, :]
                 absk2 = k1 * k1 + k2 * k2
                 B11, B12, B22 = Bh[0], Bh[1], Bh[2]
@@ -308,8 +345,8 @@ class IncompressibleEuler:
                 nu = self.visc.eval(np.sqrt(absk2))
                 out = np.stack([-L1 + nu * u_hat[0], -L2 + nu * u_hat[1]])
             else:
-                Bh = B_hat[:, iB][:, :, iB, : g.Nf]
-                k1 = k_full[:, None, None]
+                Bh = B_hat[:, iB_rows][:, :, iB, : g.Nf]
+                k1 = k_rows[:, None, None]
                 k2 = k_full[None, :, None]
                 k3 = k_last[None, None, :]
                 absk2 = k1 * k1 + k2 * k2 + k3 * k3

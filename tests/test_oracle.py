@@ -427,3 +427,14 @@ def test_golden_fixture_2d():
     sim.simulate_until(1.0, max_steps=5)
     assert np.max(np.abs(sim.u - g["u5"])) <= 1e-10 * np.max(np.abs(g["u5"]))
     assert np.allclose(sim.dts, g["dts"], rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("dim,N", [(2, 64), (3, 32)])
+def test_threaded_rows_give_identical_dudt(dim, N):
+    """compute_B / compute_dudt split over row chunks on several threads (the reference's
+    OpenMP loops, used by the CPU baseline) give exactly the single-threaded result."""
+    u = O.initialize(I.fractional_brownian(dim, N, seed=3))
+    a = O.IncompressibleEuler(O.Grid(dim, N), O.baseline_visc(N)).dudt(u)
+    for threads in (2, 5):
+        b = O.IncompressibleEuler(O.Grid(dim, N), O.baseline_visc(N), threads=threads).dudt(u)
+        assert np.array_equal(a, b), threads
