@@ -64,6 +64,7 @@ SIGNATURES = {
     "azb_workspace_bytes": (_sz, [_vp]),
     "azb_stream": (_vp, [_vp]),
     "azb_dudt": (_i, [_vp, _vp, _vp]),
+    "azb_dudt_noalloc": (_i, [_i, _i, _i, SVParams, _vp, _vp, _vp]),
     "azb_dt": (_i, [_vp, _d, _dp]),
     "azb_u_max": (_i, [_vp, _dp]),
     "azb_rk3_step": (_i, [_vp, _vp, _d, _d, _dp]),
@@ -93,6 +94,10 @@ SIGNATURES = {
     "azb_autotune": (_i, [_vp, _vp, _i]),
     "azb_set_tuning": (_i, [_vp, _ip, _i]),
     "azb_get_tuning": (_i, [_vp, _ip, _ip]),
+    "azb_num_plans": (_i, [_vp]),
+    "azb_set_plans": (_i, [_vp, _ip]),
+    "azb_get_plans": (_i, [_vp, _ip]),
+    "azb_autotune_plans": (_i, [_vp, _vp, _i]),
     "azb_synchronize": (_i, [_vp]),
     "azb_device_synchronize": (_i, []),
 }

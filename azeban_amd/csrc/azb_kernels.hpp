@@ -60,6 +60,19 @@ template <> struct LenPlan<256> { static constexpr int E = 16; using P = Plan<16
 template <> struct LenPlan<512> { static constexpr int E = 16; using P = Plan<8, 8, 8>; };
 template <> struct LenPlan<1024> { static constexpr int E = 16; using P = Plan<16, 8, 8>; };
 
+// Radix plans of the column passes: V = 0 is LenPlan's, the others are the
+// alternatives the autotuner may pick (same E, so the same thread layout and LDS
+// exchange; different butterflies, twiddles and rounding).
+template <int L, int V>
+struct ColPlan { using P = typename LenPlan<L>::P; };
+template <> struct ColPlan<384, 1> { using P = Plan<12, 4, 8>; };
+template <> struct ColPlan<384, 2> { using P = Plan<4, 4, 24>; };
+template <> struct ColPlan<768, 1> { using P = Plan<12, 8, 8>; };
+template <> struct ColPlan<768, 2> { using P = Plan<4, 8, 24>; };
+template <> struct ColPlan<1536, 1> { using P = Plan<24, 8, 8>; };
+template <> struct ColPlan<1536, 2> { using P = Plan<8, 24, 8>; };
+constexpr int col_plan_count(int L) { return (L == 384 || L == 768 || L == 1536) ? 3 : 1; }
+
 template <int... Rs>
 constexpr int first_radix(Plan<Rs...>) { return PlanFirst<Rs...>::value; }
 template <int... Rs>
@@ -108,16 +121,17 @@ __device__ __forceinline__ unsigned xcd_contiguous(unsigned b, unsigned nb) {
   return x * q + (x < r ? x : r) + i;
 }
 
-template <typename Tin, typename Tout, typename T, int L, int TS, int DIR, int MODE>
+template <typename Tin, typename Tout, typename T, int L, int TS, int DIR, int MODE, int V = 0>
 __global__ __launch_bounds__(TS * L / LenPlan<L>::E, (sizeof(T) == 4 ? 4 : (L == 768 || L == 384 ? 3 : 2))) void col_fft_kernel(ColArgs a) {
   using LP = LenPlan<L>;
+  using PP = typename ColPlan<L, V>::P;
   constexpr int E = LP::E;
   constexpr int NT = TS * L / E;
   // exchange in one barrier pair unless the interleaved buffer would exceed 64 KB
   constexpr bool SPLIT = TS * L * 2 * (int)sizeof(T) > 65536;
   using F = BlockFFT2<T, L, TS, NT, DIR, SPLIT, 1, MAP_STD, MAP_STD>;
-  constexpr int R0 = first_radix(typename LP::P{});
-  constexpr int RL = last_radix(typename LP::P{});
+  constexpr int R0 = first_radix(PP{});
+  constexpr int RL = last_radix(PP{});
   __shared__ T lds[ExchangeSize<T, L, TS, SPLIT>::words];
   const int t = threadIdx.x;
   const unsigned blk = xcd_contiguous(blockIdx.x, gridDim.x);
@@ -156,7 +170,7 @@ __global__ __launch_bounds__(TS * L / LenPlan<L>::E, (sizeof(T) == 4 ? 4 : (L ==
     }
     return r;
   });
-  F::run(v, t, lds, tw, typename LP::P{});
+  F::run(v, t, lds, tw, PP{});
   F::template store<RL>(v, t, [&](unsigned s, unsigned pos, cx<T> val) {
     unsigned row = pos;
     bool valid = s < ncols_left;

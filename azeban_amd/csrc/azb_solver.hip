@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -90,32 +91,44 @@ int make_twiddles(int L, void** out) {
 // ---------------------------------------------------------------------------
 // launch dispatch over the compile-time transform length
 // ---------------------------------------------------------------------------
-template <typename Tin, typename Tout, typename T, int L, int TS, int DIR, int MODE>
+template <typename Tin, typename Tout, typename T, int L, int TS, int DIR, int MODE, int V = 0>
 void launch_col_TS(ColArgs a, long batches, hipStream_t st) {
   constexpr int E = LenPlan<L>::E;
   a.ktiles = (a.nk + TS - 1) / TS;
   const long blocks = batches * a.nq * a.ktiles;
-  hipLaunchKernelGGL((col_fft_kernel<Tin, Tout, T, L, TS, DIR, MODE>), dim3((unsigned)blocks),
+  hipLaunchKernelGGL((col_fft_kernel<Tin, Tout, T, L, TS, DIR, MODE, V>), dim3((unsigned)blocks),
                      dim3(TS * L / E), 0, st, a);
 }
 
 // tsdiv in {1, 2, 4} selects tiles of TS0, TS0/2 or TS0/4 columns, TS0 being one
 // 128-byte line: narrower tiles give small grids more workgroups
-template <typename Tin, typename Tout, typename T, int L, int DIR, int MODE>
-void launch_col_L(ColArgs a, long batches, hipStream_t st, int tsdiv = 1) {
+template <typename Tin, typename Tout, typename T, int L, int DIR, int MODE, int V>
+void launch_col_LV(ColArgs a, long batches, hipStream_t st, int tsdiv) {
   constexpr int TS0 = azb::col_ts<T, L>();
   constexpr int TS1 = TS0 / 2 >= 1 ? TS0 / 2 : 1;
   constexpr int TS2 = TS0 / 4 >= 1 ? TS0 / 4 : 1;
   if (tsdiv >= 4)
-    launch_col_TS<Tin, Tout, T, L, TS2, DIR, MODE>(a, batches, st);
+    launch_col_TS<Tin, Tout, T, L, TS2, DIR, MODE, V>(a, batches, st);
   else if (tsdiv >= 2)
-    launch_col_TS<Tin, Tout, T, L, TS1, DIR, MODE>(a, batches, st);
+    launch_col_TS<Tin, Tout, T, L, TS1, DIR, MODE, V>(a, batches, st);
   else
-    launch_col_TS<Tin, Tout, T, L, TS0, DIR, MODE>(a, batches, st);
+    launch_col_TS<Tin, Tout, T, L, TS0, DIR, MODE, V>(a, batches, st);
+}
+
+// plan in [0, col_plan_count(L)) selects the radix plan (ColPlan)
+template <typename Tin, typename Tout, typename T, int L, int DIR, int MODE>
+void launch_col_L(ColArgs a, long batches, hipStream_t st, int tsdiv, int plan) {
+  if constexpr (col_plan_count(L) > 1) {
+    if (plan == 1) return launch_col_LV<Tin, Tout, T, L, DIR, MODE, 1>(a, batches, st, tsdiv);
+    if (plan == 2) return launch_col_LV<Tin, Tout, T, L, DIR, MODE, 2>(a, batches, st, tsdiv);
+  }
+  launch_col_LV<Tin, Tout, T, L, DIR, MODE, 0>(a, batches, st, tsdiv);
 }
 
 template <typename Tin, typename Tout, typename T, int DIR, int MODE>
-int launch_col(int L, ColArgs a, long batches, hipStream_t st, int tsdiv = 1) {
+int launch_col(int L, ColArgs a, long batches, hipStream_t st, int tsdiv = 1, int plan = 0) {
+  if (plan < 0 || plan >= col_plan_count(L))
+    return fail(AZB_ERR_ARG, "radix plan " + std::to_string(plan) + " not built for length " + std::to_string(L));
   // the kernels take the pruned length as 2L/3 and address a tile with 32-bit
   // element offsets from its base
   if (MODE != COL_FULL && 3L * a.nsmall != 2L * L)
@@ -124,7 +137,7 @@ int launch_col(int L, ColArgs a, long batches, hipStream_t st, int tsdiv = 1) {
     return fail(AZB_ERR_UNSUPPORTED, "column strides exceed 32-bit element offsets");
   switch (L) {
 #define AZB_CASE(LL) \
-  case LL: launch_col_L<Tin, Tout, T, LL, DIR, MODE>(a, batches, st, tsdiv); break;
+  case LL: launch_col_L<Tin, Tout, T, LL, DIR, MODE>(a, batches, st, tsdiv, plan); break;
     AZB_CASE(96) AZB_CASE(192) AZB_CASE(384) AZB_CASE(768) AZB_CASE(1536)
 #undef AZB_CASE
     default:
@@ -210,6 +223,8 @@ struct azb_solver {
   // tuning: column tile divisor per kernel family, line-pair divisor of the row kernel
   int tsdiv[AZB_PROF_FAMILIES] = {1, 1, 1, 1, 1, 1};
   int pdiv = 1;
+  // radix plan per column-pass family (ColPlan index; the row and k-space entries are 0)
+  int plan[AZB_PROF_FAMILIES] = {0, 0, 0, 0, 0, 0};
   DevState* probe_host = nullptr;  // pinned mirror of the device state (non-blocking probes)
   hipEvent_t probe_ev = nullptr;
   // per-kernel-family hipEvent timing (off unless enabled)
@@ -281,7 +296,7 @@ int enqueue_rhs_T(azb_solver* s, const double2* u_in, const double2* u0, double2
     c.bps = 3;
     {
       ProfScope p(s, AZB_PROF_INV_AX0);
-      if ((rc = launch_col<double, T, T, +1, COL_PAD_IN>(M, c, 3L * S, st, s->tsdiv[AZB_PROF_INV_AX0]))) return rc;
+      if ((rc = launch_col<double, T, T, +1, COL_PAD_IN>(M, c, 3L * S, st, s->tsdiv[AZB_PROF_INV_AX0], s->plan[AZB_PROF_INV_AX0]))) return rc;
     }
     // K2: inverse along axis 1; X[c][i'][j][hp] -> Y[c][i'][j'][hp]
     c.in = s->X; c.out = s->Y; c.nq = 1;
@@ -291,7 +306,7 @@ int enqueue_rhs_T(azb_solver* s, const double2* u_in, const double2* u0, double2
     c.bps = 3 * M;
     {
       ProfScope p(s, AZB_PROF_INV_AX1);
-      if ((rc = launch_col<T, T, T, +1, COL_PAD_IN>(M, c, 3L * M * S, st, s->tsdiv[AZB_PROF_INV_AX1]))) return rc;
+      if ((rc = launch_col<T, T, T, +1, COL_PAD_IN>(M, c, 3L * M * S, st, s->tsdiv[AZB_PROF_INV_AX1], s->plan[AZB_PROF_INV_AX1]))) return rc;
     }
     // K3: last axis c2r, products, max, r2c; Y[c][line][hp] -> Z[m][line][hp]
     r.in = s->Y; r.out = s->Z; r.nlines = (long)M * M;
@@ -308,7 +323,7 @@ int enqueue_rhs_T(azb_solver* s, const double2* u_in, const double2* u0, double2
     c.bps = 6 * M;
     {
       ProfScope p(s, AZB_PROF_FWD_AX1);
-      if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 6L * M * S, st, s->tsdiv[AZB_PROF_FWD_AX1]))) return rc;
+      if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 6L * M * S, st, s->tsdiv[AZB_PROF_FWD_AX1], s->plan[AZB_PROF_FWD_AX1]))) return rc;
     }
     // K5: forward along axis 0; X[m][i'][j][hp] -> Y[m][i][j][hp] (B_hat on the N-grid)
     c.in = s->X; c.out = s->Y; c.nq = N;
@@ -318,7 +333,7 @@ int enqueue_rhs_T(azb_solver* s, const double2* u_in, const double2* u0, double2
     c.bps = 6;
     {
       ProfScope p(s, AZB_PROF_FWD_AX0);
-      if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 6L * S, st, s->tsdiv[AZB_PROF_FWD_AX0]))) return rc;
+      if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 6L * S, st, s->tsdiv[AZB_PROF_FWD_AX0], s->plan[AZB_PROF_FWD_AX0]))) return rc;
     }
   } else {
     // K1: inverse along axis 0; state [c][i][k] -> X[c][i'][hp]
@@ -329,7 +344,7 @@ int enqueue_rhs_T(azb_solver* s, const double2* u_in, const double2* u0, double2
     c.bps = 2;
     {
       ProfScope p(s, AZB_PROF_INV_AX0);
-      if ((rc = launch_col<double, T, T, +1, COL_PAD_IN>(M, c, 2L * S, st, s->tsdiv[AZB_PROF_INV_AX0]))) return rc;
+      if ((rc = launch_col<double, T, T, +1, COL_PAD_IN>(M, c, 2L * S, st, s->tsdiv[AZB_PROF_INV_AX0], s->plan[AZB_PROF_INV_AX0]))) return rc;
     }
     // K2: last axis; X[c][i'][hp] -> Z[m][i'][hp]
     r.in = s->X; r.out = s->Z; r.nlines = M;
@@ -346,7 +361,7 @@ int enqueue_rhs_T(azb_solver* s, const double2* u_in, const double2* u0, double2
     c.bps = 3;
     {
       ProfScope p(s, AZB_PROF_FWD_AX0);
-      if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 3L * S, st, s->tsdiv[AZB_PROF_FWD_AX0]))) return rc;
+      if ((rc = launch_col<T, T, T, -1, COL_TRUNC_OUT>(M, c, 3L * S, st, s->tsdiv[AZB_PROF_FWD_AX0], s->plan[AZB_PROF_FWD_AX0]))) return rc;
     }
   }
   const double Md = std::pow((double)M, s->dim);
@@ -391,16 +406,7 @@ int enqueue_rk3_stages(azb_solver* s, double2* u, double C) {
   return AZB_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-const char* azb_last_error(void) { return g_err.c_str(); }
-
-int azb_create_batch(int dim, int N, int M, int n_vars, int precision, azb_sv_params visc, int nsamples,
-                     void* stream, azb_solver** out) {
-  if (!out) return fail(AZB_ERR_ARG, "out is NULL");
-  *out = nullptr;
+int check_solver_args(int dim, int N, int M, int n_vars, int precision, int nsamples) {
   if (dim != 2 && dim != 3) return fail(AZB_ERR_ARG, "dim must be 2 or 3");
   if (n_vars != dim) return fail(AZB_ERR_UNSUPPORTED, "n_vars must equal dim (no tracer)");
   if (N < 8 || N % 4 != 0) return fail(AZB_ERR_ARG, "N must be a multiple of 4");
@@ -412,13 +418,23 @@ int azb_create_batch(int dim, int N, int M, int n_vars, int precision, azb_sv_pa
   if (dim == 3 && M == 1536) return fail(AZB_ERR_UNSUPPORTED, "3D N = 1024 is not built");
   if (precision != AZB_FP64 && precision != AZB_FP32_MIXED)
     return fail(AZB_ERR_ARG, "unknown precision mode");
-  azb_solver* s = new azb_solver();
+  return AZB_OK;
+}
+
+// bytes of the intermediates X, Y, Z and of one RK stage buffer for S samples
+struct Layout {
+  size_t xs, ys, zs, us;
+};
+
+// fill a solver's grid fields (arguments already checked); no allocation
+Layout setup_solver(azb_solver* s, int dim, int N, int precision, azb_sv_params visc, int nsamples) {
+  const int M = 3 * N / 2;
   s->dim = dim;
   s->N = N;
   s->M = M;
   s->S = nsamples;
   s->nh = N / 2 + 1;
-  s->nvars = n_vars;
+  s->nvars = dim;
   s->prec = precision;
   s->visc = visc;
   // 3D: the 12-values-per-thread row kernel with 192-thread workgroups
@@ -429,19 +445,98 @@ int azb_create_batch(int dim, int N, int M, int n_vars, int precision, azb_sv_pa
   s->nhp = (nh + 15) / 16 * 16;
   const long hp = s->nhp;
   const size_t S = nsamples;
-  size_t xs, ys, zs;
+  Layout l{};
   if (dim == 3) {
     s->nmodes = (long)N * N * nh;
-    xs = S * cs * 6 * (size_t)M * N * hp;                                   // T1 (3) / S2 (6)
-    ys = S * cs * std::max<size_t>(3 * (size_t)M * M * hp, 6 * (size_t)N * N * hp);  // T2 / B_hat
-    zs = S * cs * 6 * (size_t)M * M * hp;                                   // S1
+    l.xs = S * cs * 6 * (size_t)M * N * hp;                                   // T1 (3) / S2 (6)
+    l.ys = S * cs * std::max<size_t>(3 * (size_t)M * M * hp, 6 * (size_t)N * N * hp);  // T2 / B_hat
+    l.zs = S * cs * 6 * (size_t)M * M * hp;                                   // S1
   } else {
     s->nmodes = (long)N * nh;
-    xs = S * cs * 2 * (size_t)M * hp;
-    ys = S * cs * 3 * (size_t)N * hp;
-    zs = S * cs * 3 * (size_t)M * hp;
+    l.xs = S * cs * 2 * (size_t)M * hp;
+    l.ys = S * cs * 3 * (size_t)N * hp;
+    l.zs = S * cs * 3 * (size_t)M * hp;
   }
-  const size_t us = S * sizeof(double2) * (size_t)dim * s->nmodes;
+  l.us = S * sizeof(double2) * (size_t)dim * s->nmodes;
+  return l;
+}
+
+size_t round256(size_t b) { return (b + 255) / 256 * 256; }
+
+// ---------------------------------------------------------------------------
+// Scratch of the no-allocation RHS (azb_dudt_noalloc): the twiddle tables of every
+// built padded length (FP64, then FP32) at the start, then the intermediates of one
+// solve.  Each GPU holds its own copy of the array for the life of the process.
+// ---------------------------------------------------------------------------
+constexpr int kTwLens[5] = {96, 192, 384, 768, 1536};
+constexpr size_t kScratchTwBytes = size_t(128) << 10;
+constexpr size_t kScratchBytes = size_t(96) << 20;
+
+size_t scratch_tw_offset(int precision, int M) {
+  size_t off = 0;
+  if (precision != AZB_FP64)
+    for (int L : kTwLens) off += sizeof(cx<double>) * L;
+  for (int L : kTwLens) {
+    if (L == M) break;
+    off += (precision == AZB_FP64 ? sizeof(cx<double>) : sizeof(cx<float>)) * L;
+  }
+  return off;
+}
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<unsigned char*> g_scratch_ptr;  // device address of `scratch`, per device
+
+// the current device's address of the scratch array; on a device's first use it is
+// looked up and the twiddle tables are written (the name `scratch` is a host-side
+// stand-in: only this address may reach a kernel or a copy)
+int scratch_base(unsigned char** out) {
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    unsigned char* base = (unsigned char*)p;
+    for (int L : kTwLens) {
+      const auto hd = host_twiddles<double>(L);
+      HIPCHK(hipMemcpy(base + scratch_tw_offset(AZB_FP64, L), hd.data(), sizeof(cx<double>) * L,
+                       hipMemcpyHostToDevice));
+      const auto hf = host_twiddles<float>(L);
+      HIPCHK(hipMemcpy(base + scratch_tw_offset(AZB_FP32_MIXED, L), hf.data(), sizeof(cx<float>) * L,
+                       hipMemcpyHostToDevice));
+    }
+    g_scratch_ptr[dev] = base;
+  }
+  *out = g_scratch_ptr[dev];
+  return AZB_OK;
+}
+
+static_assert(sizeof(cx<double>) * (96 + 192 + 384 + 768 + 1536) + sizeof(cx<float>) * (96 + 192 + 384 + 768 + 1536) <=
+                  kScratchTwBytes,
+              "twiddle tables must fit their region of the scratch array");
+
+}  // namespace
+
+extern "C" {
+
+const char* azb_last_error(void) { return g_err.c_str(); }
+
+int azb_create_batch(int dim, int N, int M, int n_vars, int precision, azb_sv_params visc, int nsamples,
+                     void* stream, azb_solver** out) {
+  if (!out) return fail(AZB_ERR_ARG, "out is NULL");
+  *out = nullptr;
+  int rc = check_solver_args(dim, N, M, n_vars, precision, nsamples);
+  if (rc) return rc;
+  azb_solver* s = new azb_solver();
+  const Layout lay = setup_solver(s, dim, N, precision, visc, nsamples);
+  const size_t xs = lay.xs, ys = lay.ys, zs = lay.zs, us = lay.us, S = nsamples;
   auto cleanup = [&](int code, const std::string& m) {
     azb_destroy(s);
     return fail(code, m);
@@ -452,7 +547,7 @@ int azb_create_batch(int dim, int N, int M, int n_vars, int precision, azb_sv_pa
       hipMalloc((void**)&s->st, S * sizeof(DevState)) != hipSuccess)
     return cleanup(AZB_ERR_ALLOC, "device allocation of the solver workspace failed");
   s->bytes = xs + ys + zs + 2 * us + S * sizeof(DevState);
-  int rc = precision == AZB_FP64 ? make_twiddles<double>(M, &s->tw) : make_twiddles<float>(M, &s->tw);
+  rc = precision == AZB_FP64 ? make_twiddles<double>(s->M, &s->tw) : make_twiddles<float>(s->M, &s->tw);
   if (rc) {
     std::string m = g_err;
     return cleanup(rc, m);
@@ -515,6 +610,34 @@ int azb_dudt(azb_solver* s, const void* u_hat, void* dudt_hat) {
   if (!s || !u_hat || !dudt_hat) return fail(AZB_ERR_ARG, "NULL argument");
   hipLaunchKernelGGL(step_prologue_fixed, dim3(s->S), dim3(256), 0, s->stream, s->st, 0.0);
   return enqueue_rhs(s, (const double2*)u_hat, nullptr, (double2*)dudt_hat, K_DUDT, 3, 0.0);
+}
+
+int azb_dudt_noalloc(int dim, int N, int precision, azb_sv_params visc, const void* u_hat, void* dudt_hat,
+                     void* stream) {
+  if (!u_hat || !dudt_hat) return fail(AZB_ERR_ARG, "NULL argument");
+  int rc = check_solver_args(dim, N, 0, dim, precision, 1);
+  if (rc) return rc;
+  azb_solver s;  // grid fields only: its buffers are carved out of the scratch array below
+  const Layout l = setup_solver(&s, dim, N, precision, visc, 1);
+  const size_t xs = round256(l.xs), ys = round256(l.ys), zs = round256(l.zs);
+  const size_t need = xs + ys + zs + round256(sizeof(DevState));
+  if (need > kScratchBytes - kScratchTwBytes)
+    return fail(AZB_ERR_UNSUPPORTED, std::to_string(dim) + "D N = " + std::to_string(N) + " needs " +
+                                         std::to_string(need) + " bytes of scratch; the no-allocation RHS has " +
+                                         std::to_string(kScratchBytes - kScratchTwBytes) +
+                                         " (2D N <= 1024, 3D N <= 64)");
+  unsigned char* base = nullptr;
+  if ((rc = scratch_base(&base))) return rc;
+  s.tw = base + scratch_tw_offset(precision, s.M);
+  unsigned char* w = base + kScratchTwBytes;
+  s.X = w;
+  s.Y = w + xs;
+  s.Z = w + xs + ys;
+  s.st = (DevState*)(w + xs + ys + zs);
+  s.stream = (hipStream_t)stream;
+  // clears the max|u| bins the previous call on this GPU left in the array
+  hipLaunchKernelGGL(step_prologue_fixed, dim3(1), dim3(256), 0, s.stream, s.st, 0.0);
+  return enqueue_rhs(&s, (const double2*)u_hat, nullptr, (double2*)dudt_hat, K_DUDT, 3, 0.0);
 }
 
 int azb_u_max(azb_solver* s, double* u_max_host) {
@@ -854,6 +977,72 @@ int azb_autotune(azb_solver* s, const void* u_hat, int reps) {
   (void)azb_profile_reset(s);
   if (rc) return rc;
   return azb_set_tuning(s, pick, pick_row);
+}
+
+int azb_num_plans(const azb_solver* s) { return s ? col_plan_count(s->M) : 0; }
+
+int azb_set_plans(azb_solver* s, const int* plans) {
+  if (!s || !plans) return fail(AZB_ERR_ARG, "NULL argument");
+  const int n = col_plan_count(s->M);
+  for (int f = 0; f < AZB_PROF_FAMILIES; ++f) {
+    const bool col = f != AZB_PROF_ROW && f != AZB_PROF_KSPACE;
+    if (plans[f] < 0 || plans[f] >= (col ? n : 1))
+      return fail(AZB_ERR_ARG, "radix plan " + std::to_string(plans[f]) + " of family " + std::to_string(f) +
+                                   " outside [0, " + std::to_string(col ? n : 1) + ")");
+  }
+  for (int f = 0; f < AZB_PROF_FAMILIES; ++f) s->plan[f] = plans[f];
+  return AZB_OK;
+}
+
+int azb_get_plans(const azb_solver* s, int* plans) {
+  if (!s || !plans) return fail(AZB_ERR_ARG, "NULL argument");
+  for (int f = 0; f < AZB_PROF_FAMILIES; ++f) plans[f] = s->plan[f];
+  return AZB_OK;
+}
+
+int azb_autotune_plans(azb_solver* s, const void* u_hat, int reps) {
+  if (!s || !u_hat) return fail(AZB_ERR_ARG, "NULL argument");
+  if (reps < 1) reps = 3;
+  const int n = col_plan_count(s->M);
+  if (n == 1) return AZB_OK;
+  const bool prof_was = s->prof;
+  int keep[AZB_PROF_FAMILIES], pick[AZB_PROF_FAMILIES];
+  double best[AZB_PROF_FAMILIES];
+  for (int f = 0; f < AZB_PROF_FAMILIES; ++f) {
+    keep[f] = s->plan[f];
+    pick[f] = s->plan[f];
+    best[f] = 1e300;
+  }
+  int rc = AZB_OK;
+  for (int p = 0; p < n && rc == AZB_OK; ++p) {
+    int pl[AZB_PROF_FAMILIES];
+    for (int f = 0; f < AZB_PROF_FAMILIES; ++f) pl[f] = (f == AZB_PROF_ROW || f == AZB_PROF_KSPACE) ? 0 : p;
+    if ((rc = azb_set_plans(s, pl))) break;
+    if ((rc = azb_dudt(s, u_hat, s->u1))) break;  // warm-up
+    if ((rc = azb_profile_reset(s))) break;
+    s->prof = true;
+    for (int r = 0; r < reps && rc == AZB_OK; ++r) rc = azb_dudt(s, u_hat, s->u1);
+    s->prof = false;
+    if (rc) break;
+    double ms[AZB_PROF_FAMILIES];
+    long cnt[AZB_PROF_FAMILIES];
+    if ((rc = azb_profile_read(s, ms, cnt, AZB_PROF_FAMILIES))) break;
+    for (int f = 0; f < AZB_PROF_FAMILIES; ++f) {
+      if (f == AZB_PROF_ROW || f == AZB_PROF_KSPACE || cnt[f] == 0) continue;
+      const double avg = ms[f] / cnt[f];
+      if (avg < best[f]) {
+        best[f] = avg;
+        pick[f] = p;
+      }
+    }
+  }
+  s->prof = prof_was;
+  (void)azb_profile_reset(s);
+  if (rc) {
+    (void)azb_set_plans(s, keep);
+    return rc;
+  }
+  return azb_set_plans(s, pick);
 }
 
 int azb_copy_bandwidth(double gib, int reps, double* gbs_out) {

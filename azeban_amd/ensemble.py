@@ -69,6 +69,7 @@ class Ensemble:
         self.graph = None
         self._graph_key = None
         self._t0 = [[0.0] * g.S for g in groups]  # start time of every sample in the next run()
+        self.stall_s = 600.0  # a chunk that has not finished after this long is an error
 
     def _new_stream(self) -> C.c_void_p:
         st = C.c_void_p()
@@ -88,31 +89,34 @@ class Ensemble:
         self._t0 = [[0.0] * g.S for g in self.groups]
 
     def autotune(self, reps: int = 3, cache: str | None = None) -> list:
-        """Pick each group's launch shapes (column tile widths, row line pairs) by timing
-        them on that group's real batch; `cache` stores/reuses the choice per
-        (dim, N, precision, samples)."""
-        from .solver import _tuning_load, _tuning_save
+        """Pick each group's radix plans (once per grid size and precision, shared with
+        single solves of that grid, so per-sample results stay identical to them) and
+        launch shapes (column tile widths, row line pairs, per samples count) by timing
+        them on that group's real batch; `cache` stores/reuses the choices."""
+        from .solver import _ensure_plans, _tuning_load, _tuning_save
         from ._capi import PROF_FAMILIES
 
-        store = _tuning_load(cache) if cache else {}
         out = []
         for g in self.groups:
+            torch.cuda.synchronize()
+            # radix plans per grid size, shared with single solves of that grid
+            plans = _ensure_plans(g.handle, g.dim, g.N, g.precision, g.u, reps, cache)
+            store = _tuning_load(cache) if cache else {}
             key = f"{g.dim}/{g.N}/{g.precision}/{g.S}"
             if key in store:
                 rec = store[key]
             else:
-                torch.cuda.synchronize()
                 check(self.lib.azb_autotune(g.handle, _vp(g.u), reps))
                 ts = (C.c_int * len(PROF_FAMILIES))()
                 pd = C.c_int()
                 check(self.lib.azb_get_tuning(g.handle, ts, C.byref(pd)))
                 rec = {"tsdiv": list(ts), "pdiv": pd.value}
-                store[key] = rec
+                if cache:
+                    store[key] = rec
+                    _tuning_save(cache, store)
             ts = (C.c_int * len(PROF_FAMILIES))(*rec["tsdiv"])
             check(self.lib.azb_set_tuning(g.handle, ts, rec["pdiv"]))
-            out.append(rec)
-        if cache:
-            _tuning_save(cache, store)
+            out.append(dict(rec, plans=plans))
         self._drop_graph()  # launch shapes changed: capture again
         return out
 
@@ -167,10 +171,13 @@ class Ensemble:
             t = (C.c_double * g.S)()
             n = (C.c_int * g.S)()
             ready = C.c_int(0)
+            t0 = time.perf_counter()
             while True:
                 check(self.lib.azb_loop_probe_result(g.handle, C.byref(ready), t, n))
                 if ready.value:
                     break
+                if time.perf_counter() - t0 > self.stall_s:
+                    raise RuntimeError(f"no progress of group {g.dim}D N={g.N} for {self.stall_s} s")
                 time.sleep(poll_sleep)
             out.append((list(t), list(n)))
         return out

@@ -146,6 +146,7 @@ class Server:
     chunk_steps: int = 16
     C: float = 0.2
     slots: dict = field(default_factory=dict)  # (dim, N) -> slots, overriding DEFAULT_SLOTS
+    stall_s: float = 600.0  # running classes without any progress for this long is an error
     results: list = field(default_factory=list)
 
     def _slots_for(self, dim: int, N: int) -> int:
@@ -181,6 +182,7 @@ class Server:
                 del classes[c.key]
             return used() + need <= self.hbm_budget
 
+        last_progress = 0.0
         try:
             while pending or n_active():
                 # ---- admission (FIFO): a free slot of the request's class, creating the
@@ -202,6 +204,7 @@ class Server:
                         break
                     pending.popleft()
                     self._admit(lib, cl, i, r, now())
+                    last_progress = now()
                     if not cl.probing:
                         # the class was idle: start it with two chunks, probed after the first
                         cl.chunk(self.chunk_steps)
@@ -230,6 +233,10 @@ class Server:
                         cl.probe()  # behind the chunk already queued
                         cl.chunk(self.chunk_steps)
                         launches += 1
+                if progressed:
+                    last_progress = now()
+                elif n_active() and now() - last_progress > self.stall_s:
+                    raise RuntimeError(f"no request made progress for {self.stall_s} s")
                 if not progressed:
                     if not n_active() and pending:
                         wait = pending[0].arrival - now()

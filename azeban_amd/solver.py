@@ -171,21 +171,36 @@ class IncompressibleEuler:
         ts = (C.c_int * len(_capi.PROF_FAMILIES))(*tsdiv)
         check(load().azb_set_tuning(self.handle, ts, pdiv))
 
+    # radix plans of the column passes (results change at round-off with the plan)
+    def get_plans(self):
+        pl = (C.c_int * len(_capi.PROF_FAMILIES))()
+        check(load().azb_get_plans(self.handle, pl))
+        return list(pl)
+
+    def set_plans(self, plans) -> None:
+        check(load().azb_set_plans(self.handle, (C.c_int * len(_capi.PROF_FAMILIES))(*plans)))
+
     def autotune(self, u_hat: torch.Tensor, reps: int = 3, cache: str | None = None, nsamples: int = 1):
-        """Pick launch shapes by timing on the device; with `cache`, reuse a stored choice
-        keyed by (dim, N, precision, samples) or store the new one."""
+        """Pick the radix plans, then the launch shapes, by timing on the device.  The plans
+        are chosen once per (dim, N, precision) and shared by every solver of that grid
+        in the process (and in `cache`), so that batched and single solves of one grid
+        keep computing bit-identical results; the launch shapes are chosen per
+        (dim, N, precision, samples)."""
+        _ensure_plans(self.handle, self.grid.dim, self.grid.N_phys, self.precision, u_hat, reps, cache)
         key = f"{self.grid.dim}/{self.grid.N_phys}/{self.precision}/{nsamples}"
         store = _tuning_load(cache) if cache else {}
+        plans = self.get_plans()
         if key in store:
             self.set_tuning(store[key]["tsdiv"], store[key]["pdiv"])
-            return store[key]
+            return dict(store[key], plans=plans)
         check(load().azb_autotune(self.handle, _ptr(u_hat), reps))
         ts, pd = self.get_tuning()
         rec = {"tsdiv": ts, "pdiv": pd}
         if cache:
+            store = _tuning_load(cache)
             store[key] = rec
             _tuning_save(cache, store)
-        return rec
+        return dict(rec, plans=plans)
 
     # per-kernel-family hipEvent timing on the solver's stream
     def profile_enable(self, on: bool = True) -> None:
@@ -205,6 +220,35 @@ class IncompressibleEuler:
         if tuple(u.shape) != self.grid.shape_fourier(self.n_vars()) or u.dtype != torch.complex128:
             raise ValueError(f"expected complex128 of shape {self.grid.shape_fourier(self.n_vars())}, "
                              f"got {u.dtype} {tuple(u.shape)}")
+
+
+# radix plans per (dim, N, precision), shared by every solver of the process
+_PLANS: dict = {}
+
+
+def _ensure_plans(handle, dim: int, N: int, precision: str, u_hat: torch.Tensor, reps: int,
+                  cache: str | None) -> list:
+    """Give the solver the plans of its grid: from this process, from `cache`, or timed
+    now on the device (azb_autotune_plans) and remembered."""
+    lib = load()
+    key = f"{dim}/{N}/{precision}"
+    plans = _PLANS.get(key)
+    if plans is None and cache:
+        plans = _tuning_load(cache).get("plans", {}).get(key)
+    n = len(_capi.PROF_FAMILIES)
+    if plans is None:
+        check(lib.azb_autotune_plans(handle, C.c_void_p(u_hat.data_ptr()), reps))
+        pl = (C.c_int * n)()
+        check(lib.azb_get_plans(handle, pl))
+        plans = list(pl)
+        if cache:
+            store = _tuning_load(cache)
+            store.setdefault("plans", {})[key] = plans
+            _tuning_save(cache, store)
+    else:
+        check(lib.azb_set_plans(handle, (C.c_int * n)(*plans)))
+    _PLANS[key] = list(plans)
+    return list(plans)
 
 
 def _tuning_load(path: str) -> dict:

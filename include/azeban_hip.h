@@ -83,6 +83,18 @@ void* azb_stream(const azb_solver* s);
  * u_hat.  Also records max|u| / M^d for azb_dt. */
 int azb_dudt(azb_solver* s, const void* u_hat, void* dudt_hat);
 
+/* dudt_hat = L(u_hat) like azb_dudt, with no solver handle and no allocation: no
+ * counterpart in the reference; for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops).  The intermediates live in a
+ * file-scope device array of 96 MiB that each GPU holds until the process ends
+ * (its twiddle tables are written on the first call on a device), so only
+ * 2D N <= 1024 and 3D N <= 64 fit (larger grids: AZB_ERR_UNSUPPORTED).  Queued on
+ * `stream` (NULL = null stream).  Every call on one GPU uses the same array: calls
+ * that overlap on one GPU (other streams, other host threads) race and must be
+ * ordered by the caller.  Results are bit-identical to azb_dudt's. */
+int azb_dudt_noalloc(int dim, int N, int precision, azb_sv_params visc, const void* u_hat, void* dudt_hat,
+                     void* stream);
+
 /* C / (N u_max) of the latest RHS evaluation, IncompressibleEuler::dt
  * (incompressible_euler.hpp:122-124).  Synchronising. */
 int azb_dt(azb_solver* s, double C, double* dt_host);
@@ -165,6 +177,20 @@ int azb_profile_reset(azb_solver* s);
 int azb_autotune(azb_solver* s, const void* u_hat, int reps);
 int azb_set_tuning(azb_solver* s, const int* tsdiv, int pdiv);
 int azb_get_tuning(const azb_solver* s, int* tsdiv, int* pdiv);
+/* Radix plans of the column passes (per family; the row and k-space entries are 0).
+ * Padded lengths 384, 768 and 1536 have three: plan 0 = 8.4.12 / 8.8.12 / 8.8.24,
+ * plan 1 = 12.4.8 / 12.8.8 / 24.8.8, plan 2 = 4.4.24 / 4.8.24 / 8.24.8; the other
+ * lengths one.  Unlike the launch shapes, the plan changes the rounding (results
+ * agree to round-off, not bit for bit), so solves that must agree bit for bit (a
+ * batch and its single solves) use the same plans.  azb_autotune_plans times each
+ * plan on the device per family (reps RHS evaluations into the solver's own
+ * buffers) and keeps the fastest.  The reference's nearest counterpart is its FFT
+ * size choice from a benchmark file (src/azeban/operations/fft.cpp:63-108).
+ * Synchronising. */
+int azb_num_plans(const azb_solver* s);
+int azb_set_plans(azb_solver* s, const int* plans);
+int azb_get_plans(const azb_solver* s, int* plans);
+int azb_autotune_plans(azb_solver* s, const void* u_hat, int reps);
 
 /* ---- standalone operations (test seams) -------------------------------------- */
 

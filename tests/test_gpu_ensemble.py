@@ -87,6 +87,9 @@ def test_autotuned_shapes_give_identical_results(cuda, dim, N):
         assert torch.equal(out, ref), (ts, pd)
     rec = eq.autotune(u, reps=2)
     assert all(v in (1, 2, 4) for v in rec["tsdiv"]) and rec["pdiv"] in (1, 2)
+    assert all(0 <= p < 3 for p in rec["plans"]) and rec["plans"][2] == rec["plans"][5] == 0
+    # the tuned launch shapes with the default radix plans: identical to the defaults
+    eq.set_plans([0] * 6)
     out = torch.empty_like(u)
     eq.dudt(out, u)
     assert torch.equal(out, ref)

@@ -309,6 +309,48 @@ def test_dudt_3d_512_broadband_against_oracle(cuda, broadband_512, broadband_512
     assert abs(eq.u_max() - u_max) <= (1e-14 if precision == "fp64" else 1e-6) * u_max
 
 
+@pytest.mark.parametrize("plan", [1, 2])
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-12), ("fp32_mixed", 1e-5)])
+def test_radix_plans_3d_512_broadband_against_oracle(cuda, broadband_512, broadband_512_rhs, precision, tol,
+                                                     plan):
+    """Every radix plan the autotuner may select for the 768-point column passes
+    (12.8.8, 4.8.24 beside the default 8.8.12), at full size against the FP64 oracle."""
+    N = 512
+    print(f"[case] 3D broadband dudt N={N} {precision} radix plan {plan}", flush=True)
+    ref, _ = broadband_512_rhs
+    g, eq, _ = make_pair(3, N, precision)
+    eq.set_plans([plan, plan, 0, plan, plan, 0])
+    u = dev(broadband_512)
+    out = torch.empty_like(u)
+    eq.dudt(out, u)
+    got = host(out)
+    del u, out
+    err = rel(got, ref)
+    print(f"3D broadband N={N} {precision} plan {plan}: rel L2 {err:.3e}", flush=True)
+    assert err <= tol, err
+
+
+@pytest.mark.parametrize("dim,N", [(2, 256), (2, 512), (2, 1024), (3, 256)])
+@pytest.mark.parametrize("plan", [0, 1, 2])
+def test_radix_plans_against_oracle(cuda, dim, N, plan):
+    """Each radix plan of the 384-, 768- and 1536-point column passes against the
+    oracle (FP64, 1e-12); the mixed plans per family as the autotuner may pick them."""
+    print(f"[case] dudt {dim}D N={N} radix plan {plan}", flush=True)
+    uh = random_state(dim, N)
+    g, eq, oeq = make_pair(dim, N)
+    if dim == 3:
+        oeq = O.IncompressibleEuler(O.Grid(3, N), O.baseline_visc(N), workers=FFT_WORKERS)
+    ref = oeq.dudt(uh)
+    u = dev(uh)
+    for plans in ([plan, plan, 0, plan, plan, 0], [plan, (plan + 1) % 3, 0, (plan + 2) % 3, plan, 0]):
+        eq.set_plans(plans)
+        out = torch.empty_like(u)
+        eq.dudt(out, u)
+        err = rel(host(out), ref)
+        assert err <= 1e-12, (plans, err)
+        assert eq.get_plans() == plans
+
+
 def test_rk3_step_3d_512_broadband_against_oracle(cuda, broadband_512):
     """One SSP-RK3 step at 512^3 (three RHS, the fused stage updates, the device CFL
     step) against the oracle's step from the same broadband state."""
