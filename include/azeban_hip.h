@@ -89,9 +89,12 @@ int azb_dudt(azb_solver* s, const void* u_hat, void* dudt_hat);
  * file-scope device array of 96 MiB that each GPU holds until the process ends
  * (its twiddle tables are written on the first call on a device), so only
  * 2D N <= 1024 and 3D N <= 64 fit (larger grids: AZB_ERR_UNSUPPORTED).  Queued on
- * `stream` (NULL = null stream).  Every call on one GPU uses the same array: calls
- * that overlap on one GPU (other streams, other host threads) race and must be
- * ordered by the caller.  Results are bit-identical to azb_dudt's. */
+ * `stream` (NULL = null stream).  The first call on a device looks up the array's
+ * address and uploads the twiddle tables with a synchronous copy, so it must not
+ * be made under graph capture; later calls only launch kernels.  Every call on one
+ * GPU uses the same array: calls that overlap on one GPU (other streams, other host
+ * threads) race and must be ordered by the caller.  Results are bit-identical to
+ * azb_dudt's. */
 int azb_dudt_noalloc(int dim, int N, int precision, azb_sv_params visc, const void* u_hat, void* dudt_hat,
                      void* stream);
 
