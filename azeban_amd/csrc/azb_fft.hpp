@@ -18,6 +18,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace azb {
 
 template <typename T>
@@ -474,13 +476,18 @@ namespace azb {
 //             of wave w hold 32 w + l, lanes 32..63 the mirrors (wave 0 pairs the
 //             two self-mirrored butterflies 0 and NB/2 in lanes 0 and 32)
 //   MAP_WAVE: wave w owns sequence w (NT = 64 TS and L = 64 E), lane l its
-//             butterflies l + 64 b; an exchange between two passes that keep every
-//             sequence inside its own wave (MAP_WAVE, or MAP_SEQ1 with L/R = 64)
-//             moves data only within a wave and needs no workgroup barrier
-// Any mapping can be used by any pass: the LDS exchange writes with the mapping of
-// the pass that produced the data and reads with the mapping of the next pass.
+//             butterflies l + 64 b
+//   MAP_HALF: half-wave h (32 consecutive threads) owns sequence h (NT = 32 TS and
+//             L = 32 E), lane l of it the butterflies l + 32 b, so wave w holds
+//             sequences 2w and 2w + 1
+// An exchange between two passes that keep every sequence inside the same wave
+// (MAP_WAVE, or MAP_SEQ1 with L/R = 64: wave w holds sequence w; MAP_HALF, or
+// MAP_SEQ1 with L/R = 32: wave w holds sequences 2w, 2w + 1) moves data only
+// within a wave and needs no workgroup barrier.  Any mapping can be used by any
+// pass: the LDS exchange writes with the mapping of the pass that produced the
+// data and reads with the mapping of the next pass.
 // ---------------------------------------------------------------------------
-enum { MAP_STD = 0, MAP_SEQ = 1, MAP_SEQ1 = 3, MAP_MIRROR = 4, MAP_WAVE = 5 };
+enum { MAP_STD = 0, MAP_SEQ = 1, MAP_SEQ1 = 3, MAP_MIRROR = 4, MAP_WAVE = 5, MAP_HALF = 6 };
 
 // Index arithmetic is unsigned throughout: signed division and modulo by a power of
 // two cost a sign fix-up each, and these run once per element of every exchange.
@@ -499,6 +506,10 @@ __device__ __forceinline__ void bmap(unsigned t, unsigned b, unsigned& s, unsign
     static_assert(NT % 64 == 0, "whole waves");
     s = t / 64u;
     j = t % 64u + 64u * b;
+  } else if constexpr (MAP == MAP_HALF) {
+    static_assert(NT == 32 * TS, "one half-wave per sequence");
+    s = t / 32u;
+    j = t % 32u + 32u * b;
   } else if constexpr (MAP == MAP_SEQ || MAP == MAP_SEQ1) {
     constexpr unsigned GG = MAP == MAP_SEQ1 ? 1 : G;
     const unsigned q = b / GG, gg = b % GG;
@@ -546,26 +557,83 @@ struct ExLayout {
   static constexpr unsigned delta(int D) { return SWZ ? D + D / PADE : D * TS; }
 };
 
-// MIDW: bit P set = middle pass P uses MAP_WAVE instead of MAP_STD.  PADE: pad
-// period of the swizzled layout in elements (0: one 16-byte complex per slot).
+// Sequence-major layout with an XOR swizzle and no pad slots: element (s, pos) at
+// s*LP + (pos ^ (((pos >> A) & MASK) << S)).  The swizzle permutes positions inside
+// aligned blocks, so it is a bijection on [0, L) that stays inside the sequence's
+// LP slots; a position step that is a multiple of (MASK+1) << A leaves the swizzle
+// term unchanged, so such steps are constant address offsets.
+template <typename T, int L, int TS, int A, int MASK, int S, int LP_>
+struct XorLayout {
+  // S < A: each swizzled bit takes its term from a higher bit, so the map can be
+  // undone from the top bit down (a bijection); S = A would clear bits instead
+  static_assert((MASK & (MASK + 1)) == 0 && S < A && L % ((MASK + 1) << A) == 0 && LP_ >= L,
+                "swizzle blocks must tile a sequence");
+  static constexpr int LP = LP_;
+  static constexpr int elems = TS * LP;
+  __device__ __forceinline__ static unsigned at(unsigned s, unsigned pos) {
+    return s * (unsigned)LP + (pos ^ (((pos >> A) & (unsigned)MASK) << S));
+  }
+  static constexpr bool linear(int D) { return D % ((MASK + 1) << A) == 0; }
+  static constexpr unsigned delta(int D) { return D; }
+};
+
+// Layout of exchange P (the one after pass P): LSEL::lay<P> when a selector is
+// given, else the transform's default layout D.
+template <class LSEL, int P, class D>
+struct LaySel {
+  using type = typename LSEL::template lay<P>;
+};
+template <int P, class D>
+struct LaySel<void, P, D> {
+  using type = D;
+};
+template <class LSEL, class D>
+struct LayElems {
+  static constexpr int value = LSEL::elems > D::elems ? LSEL::elems : D::elems;
+};
+template <class D>
+struct LayElems<void, D> {
+  static constexpr int value = D::elems;
+};
+
+// MIDW / MIDH: bit P set = middle pass P uses MAP_WAVE / MAP_HALF instead of
+// MAP_STD.  PADE: pad period of the swizzled layout in elements (0: one 16-byte
+// complex per slot).  LSEL: per-exchange layouts (void: ExLayout for every
+// exchange); every layout of one transform must give each sequence the same LP
+// slots, so that a wave-local exchange writes only where the previous exchange's
+// readers of its own sequences read.
+// PTAB: tw is this transform's per-pass twiddle table (pass_table_entries /
+// fill_pass_table: for each pass P >= 1 and each of its Ns twiddle classes k one
+// row of R entries, W^{r k L/(Ns R)} with the transform's direction applied, for
+// r = 1 .. R-1 at row[r-1]) instead of the W_L^e table: one base address per
+// butterfly, constant offsets, 16-byte loads of two FP32 entries.
 template <typename T, int L, int TS, int NT, int DIR, bool SPLIT, int G, int MAPF, int MAPL, bool SWZ = false,
-          int SOFF = 1, unsigned MIDW = 0, int PADE = 0>
+          int SOFF = 1, unsigned MIDW = 0, int PADE = 0, unsigned MIDH = 0, class LSEL = void, bool PTAB = false>
 struct BlockFFT2 {
   static constexpr int E = TS * L / NT;
   static_assert(TS * L % NT == 0, "TS*L must be a multiple of NT");
   static_assert(MIDW == 0 || (NT == 64 * TS && L == 64 * E), "MAP_WAVE needs one wave per sequence");
+  static_assert(MIDH == 0 || (NT == 32 * TS && L == 32 * E), "MAP_HALF needs one half-wave per sequence");
+  static_assert((MIDW & MIDH) == 0, "one mapping per pass");
   using LY = ExLayout<T, L, TS, SWZ, SOFF, PADE>;
+  template <int P>
+  using XL = typename LaySel<LSEL, P, LY>::type;
   // LDS words (of T) the exchange needs
-  static constexpr int lds_words = SPLIT ? LY::elems : 2 * LY::elems;
+  static constexpr int lds_words = SPLIT ? LayElems<LSEL, LY>::value : 2 * LayElems<LSEL, LY>::value;
 
   template <int P, int NP>
   static constexpr int map_of() {
-    return P == 0 ? MAPF : (P == NP - 1 ? MAPL : (((MIDW >> P) & 1u) ? MAP_WAVE : MAP_STD));
+    return P == 0 ? MAPF
+                  : (P == NP - 1 ? MAPL
+                                 : (((MIDW >> P) & 1u) ? MAP_WAVE : (((MIDH >> P) & 1u) ? MAP_HALF : MAP_STD)));
   }
-  // every sequence stays inside its own wave (sequence s = wave s) under this mapping
+  // sequences per wave when this mapping keeps every sequence inside one wave
+  // (wave w holds sequences spw w ... spw w + spw - 1), else 0
   template <int MAP, int R>
-  static constexpr bool wave_local() {
-    return NT == 64 * TS && (MAP == MAP_WAVE || (MAP == MAP_SEQ1 && L / R == 64));
+  static constexpr int wave_owner() {
+    if (MAP == MAP_WAVE || (MAP == MAP_SEQ1 && L / R == 64 && E == R)) return NT == 64 * TS ? 1 : 0;
+    if (MAP == MAP_HALF || (MAP == MAP_SEQ1 && L / R == 32 && E == R)) return NT == 32 * TS ? 2 : 0;
+    return 0;
   }
   // Between the write and the read phases of an exchange, and before the writes
   // (earlier reads of the same words): a workgroup barrier, or, when both mappings
@@ -612,7 +680,20 @@ struct BlockFFT2 {
         bmap<MAP, TS, NT, L, R, G>(t, b, s, j);
         const unsigned k = j % Ns;
         constexpr unsigned step = L / (Ns * R);
-        if (sizeof(T) == 8) {
+        if constexpr (PTAB) {
+          static_assert(sizeof(T) == 4 && R % 2 == 0, "per-pass tables: FP32, even radix");
+          // row k of this pass's table, 16-byte aligned (rows of R entries); entries
+          // r = 1 .. R-1 at row[r-1], loaded two at a time
+          typedef float f4 __attribute__((ext_vector_type(4)));
+          const __attribute__((address_space(1))) f4* row =
+              (const __attribute__((address_space(1))) f4*)(tw + k * (unsigned)R);
+#pragma unroll
+          for (int q = 0; q < R / 2; ++q) {
+            const f4 two = row[q];
+            w[2 * q + 1] = cmul(w[2 * q + 1], mk(T(two.x), T(two.y)));
+            if (2 * q + 2 < R) w[2 * q + 2] = cmul(w[2 * q + 2], mk(T(two.z), T(two.w)));
+          }
+        } else if (sizeof(T) == 8) {
           // FP64: one table load per butterfly, the other powers by repeated
           // multiplication (w^(r+1) = w^r w, below 22 ulp at R = 24).  Each saved
           // load is a full vector-memory instruction through the texture path; the
@@ -637,10 +718,11 @@ struct BlockFFT2 {
     }
   }
 
-  template <int MAPW, int MAPR, int Ns, int R, int Rn>
+  template <int MAPW, int MAPR, int Ns, int R, int Rn, class XLY = LY>
   __device__ __forceinline__ static void exchange(cx<T>* v, int t_in, T* lds) {
     constexpr int comps = SPLIT ? 2 : 1;
-    constexpr bool local = wave_local<MAPW, R>() && wave_local<MAPR, Rn>();
+    constexpr bool local = wave_owner<MAPW, R>() != 0 && wave_owner<MAPW, R>() == wave_owner<MAPR, Rn>();
+    static_assert(XLY::LP == LY::LP, "every exchange layout of a transform needs the same sequence stride");
     // opaque copy of the thread index: LDS addresses are recomputed here instead of
     // being hoisted (they are loop-invariant across transforms) and held live
     unsigned t = t_in;
@@ -655,10 +737,10 @@ struct BlockFFT2 {
         bmap<MAPW, TS, NT, L, R, G>(t, b, s, j);
         const unsigned k = j % Ns;
         const unsigned base = (j - k) * R + k;
-        const unsigned a0 = LY::at(s, base);
+        const unsigned a0 = XLY::at(s, base);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
-          const unsigned idx = LY::linear(Ns) ? a0 + r * LY::delta(Ns) : LY::at(s, base + r * Ns);
+          const unsigned idx = XLY::linear(Ns) ? a0 + r * XLY::delta(Ns) : XLY::at(s, base + r * Ns);
           if (SPLIT) {
             lds[idx] = c == 0 ? v[b * R + r].x : v[b * R + r].y;
           } else {
@@ -672,10 +754,10 @@ struct BlockFFT2 {
       for (int b = 0; b < E / Rn; ++b) {
         unsigned s, j;
         bmap<MAPR, TS, NT, L, Rn, G>(t, b, s, j);
-        const unsigned a0 = LY::at(s, j);
+        const unsigned a0 = XLY::at(s, j);
 #pragma unroll
         for (int r = 0; r < Rn; ++r) {
-          const unsigned idx = LY::linear(L / Rn) ? a0 + r * LY::delta(L / Rn) : LY::at(s, j + r * (L / Rn));
+          const unsigned idx = XLY::linear(L / Rn) ? a0 + r * XLY::delta(L / Rn) : XLY::at(s, j + r * (L / Rn));
           if (SPLIT) {
             if (c == 0)
               v[b * Rn + r].x = lds[idx];
@@ -697,8 +779,9 @@ struct BlockFFT2 {
   template <int P, int NP, int Ns, int R, int Rn, int... Rs>
   __device__ __forceinline__ static void passes(cx<T>* v, int t, T* lds, const cx<T>* tw, Plan<R, Rn, Rs...>) {
     butterflies<map_of<P, NP>(), Ns, R>(v, t, tw);
-    exchange<map_of<P, NP>(), map_of<P + 1, NP>(), Ns, R, Rn>(v, t, lds);
-    passes<P + 1, NP, Ns * R, Rn, Rs...>(v, t, lds, tw, Plan<Rn, Rs...>{});
+    exchange<map_of<P, NP>(), map_of<P + 1, NP>(), Ns, R, Rn, XL<P>>(v, t, lds);
+    // per-pass tables: pass P >= 1 takes Ns R entries, the next pass's start after them
+    passes<P + 1, NP, Ns * R, Rn, Rs...>(v, t, lds, (PTAB && P >= 1) ? tw + Ns * R : tw, Plan<Rn, Rs...>{});
   }
 
   // all threads of the workgroup call run (it holds barriers)
@@ -715,12 +798,12 @@ struct BlockFFT2 {
   __device__ __forceinline__ static void run_rest(cx<T>* v, int t, T* lds, const cx<T>* tw, Plan<R0, R1, Rs...>) {
     static_assert(plan_prod<R0, R1, Rs...>() == L, "plan must multiply to L");
     constexpr int NP = PlanAt<R0, R1, Rs...>::count;
-    exchange<map_of<0, NP>(), map_of<1, NP>(), 1, R0, R1>(v, t, lds);
+    exchange<map_of<0, NP>(), map_of<1, NP>(), 1, R0, R1, XL<0>>(v, t, lds);
     passes<1, NP, R0, R1, Rs...>(v, t, lds, tw, Plan<R1, Rs...>{});
   }
 
   // Continue a transform whose first pass (radix R0, Ns = 1) the caller computed
-  // and wrote to LDS itself (element (s, pos) at LY::at(s, pos), followed by a
+  // and wrote to LDS itself (element (s, pos) at XL<0>::at(s, pos), followed by a
   // barrier): read pass 1's inputs with pass 1's mapping and run passes 1 ...
   template <int R0, int R1, int... Rs>
   __device__ __forceinline__ static void run_after_first(cx<T>* v, int t_in, T* lds, const cx<T>* tw,
@@ -734,10 +817,10 @@ struct BlockFFT2 {
     for (int b = 0; b < E / R1; ++b) {
       unsigned s, j;
       bmap<map_of<1, NP>(), TS, NT, L, R1, G>(t, b, s, j);
-      const unsigned a0 = LY::at(s, j);
+      const unsigned a0 = XL<0>::at(s, j);
 #pragma unroll
       for (int r = 0; r < R1; ++r) {
-        const unsigned idx = LY::linear(L / R1) ? a0 + r * LY::delta(L / R1) : LY::at(s, j + r * (L / R1));
+        const unsigned idx = XL<0>::linear(L / R1) ? a0 + r * XL<0>::delta(L / R1) : XL<0>::at(s, j + r * (L / R1));
         v[b * R1 + r].x = lds[2 * idx];
         v[b * R1 + r].y = lds[2 * idx + 1];
       }
@@ -787,4 +870,39 @@ struct BlockFFT2 {
   }
 };
 
+
+// Per-pass twiddle table of a plan (see BlockFFT2's PTAB): entries for passes
+// P >= 1, Ns_P rows of R_P entries each.  Host side: the values are the W_L^e
+// table's (the same rounding), conjugated for the inverse direction.
+template <int R0>
+constexpr int pass_table_rest(int, Plan<R0>) { return 0; }
+template <int R0, int R1, int... Rs>
+constexpr int pass_table_rest(int ns, Plan<R0, R1, Rs...>) {
+  return ns * R0 * R1 + pass_table_rest(ns * R0, Plan<R1, Rs...>{});
+}
+template <int R0, int... Rs>
+constexpr int pass_table_entries(Plan<R0, Rs...>) {
+  return pass_table_rest(1, Plan<R0, Rs...>{});
+}
+// fill out[0 .. pass_table_entries) from the W_L^e table h (h[e] = exp(-2 pi i e / L))
+template <typename T, int R0>
+void fill_pass_rest(cx<T>*, const cx<T>*, int, int, int, Plan<R0>) {}
+template <typename T, int R0, int R1, int... Rs>
+void fill_pass_rest(cx<T>* out, const cx<T>* h, int L, int dir, int ns, Plan<R0, R1, Rs...>) {
+  const int Ns = ns * R0, R = R1, step = L / (Ns * R);
+  for (int k = 0; k < Ns; ++k)
+    for (int r = 0; r < R; ++r) {
+      cx<T> w{};
+      if (r >= 1) {
+        w = h[r * k * step];
+        if (dir > 0) w.y = -w.y;
+      }
+      out[k * R + (r >= 1 ? r - 1 : R - 1)] = w;
+    }
+  fill_pass_rest(out + Ns * R, h, L, dir, Ns, Plan<R1, Rs...>{});
+}
+template <typename T, int R0, int... Rs>
+void fill_pass_table(cx<T>* out, const cx<T>* h, int L, int dir, Plan<R0, Rs...>) {
+  fill_pass_rest(out, h, L, dir, 1, Plan<R0, Rs...>{});
+}
 }  // namespace azb

@@ -77,12 +77,34 @@ std::vector<cx<T>> host_twiddles(int L) {
   return h;
 }
 
+// The twiddle buffer of a solver or FFT of length L at precision T: the W_L^e
+// table, followed for the FP32 768-point row kernel by its per-pass tables.
+template <typename T>
+size_t twiddle_entries(int L) {
+  if (sizeof(T) == 4 && L == 768) return Row8Cfg<float, 768>::TW_END;
+  return (size_t)L;
+}
+
+template <typename T>
+std::vector<cx<T>> host_tables(int L) {
+  std::vector<cx<T>> h = host_twiddles<T>(L);
+  h.resize(twiddle_entries<T>(L));
+  if constexpr (sizeof(T) == 4) {
+    if (L == 768) {
+      using C = Row8Cfg<float, 768>;
+      fill_pass_table(h.data() + C::TW_INV, h.data(), L, +1, Row8Plan<768>::Inv{});
+      fill_pass_table(h.data() + C::TW_FWD, h.data(), L, -1, Row8Plan<768>::Fwd{});
+    }
+  }
+  return h;
+}
+
 template <typename T>
 int make_twiddles(int L, void** out) {
-  const std::vector<cx<T>> h = host_twiddles<T>(L);
+  const std::vector<cx<T>> h = host_tables<T>(L);
   DevFree buf;
-  HIPCHK(hipMalloc(&buf.p, sizeof(cx<T>) * L));
-  HIPCHK(hipMemcpy(buf.p, h.data(), sizeof(cx<T>) * L, hipMemcpyHostToDevice));
+  HIPCHK(hipMalloc(&buf.p, sizeof(cx<T>) * h.size()));
+  HIPCHK(hipMemcpy(buf.p, h.data(), sizeof(cx<T>) * h.size(), hipMemcpyHostToDevice));
   *out = buf.p;
   buf.p = nullptr;
   return AZB_OK;
@@ -475,10 +497,11 @@ constexpr size_t kScratchBytes = size_t(96) << 20;
 size_t scratch_tw_offset(int precision, int M) {
   size_t off = 0;
   if (precision != AZB_FP64)
-    for (int L : kTwLens) off += sizeof(cx<double>) * L;
+    for (int L : kTwLens) off += sizeof(cx<double>) * twiddle_entries<double>(L);
   for (int L : kTwLens) {
     if (L == M) break;
-    off += (precision == AZB_FP64 ? sizeof(cx<double>) : sizeof(cx<float>)) * L;
+    off += precision == AZB_FP64 ? sizeof(cx<double>) * twiddle_entries<double>(L)
+                                 : sizeof(cx<float>) * twiddle_entries<float>(L);
   }
   return off;
 }
@@ -505,11 +528,11 @@ int scratch_base(unsigned char** out) {
     HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
     unsigned char* base = (unsigned char*)p;
     for (int L : kTwLens) {
-      const auto hd = host_twiddles<double>(L);
-      HIPCHK(hipMemcpy(base + scratch_tw_offset(AZB_FP64, L), hd.data(), sizeof(cx<double>) * L,
+      const auto hd = host_tables<double>(L);
+      HIPCHK(hipMemcpy(base + scratch_tw_offset(AZB_FP64, L), hd.data(), sizeof(cx<double>) * hd.size(),
                        hipMemcpyHostToDevice));
-      const auto hf = host_twiddles<float>(L);
-      HIPCHK(hipMemcpy(base + scratch_tw_offset(AZB_FP32_MIXED, L), hf.data(), sizeof(cx<float>) * L,
+      const auto hf = host_tables<float>(L);
+      HIPCHK(hipMemcpy(base + scratch_tw_offset(AZB_FP32_MIXED, L), hf.data(), sizeof(cx<float>) * hf.size(),
                        hipMemcpyHostToDevice));
     }
     g_scratch_ptr[dev] = base;
@@ -518,7 +541,8 @@ int scratch_base(unsigned char** out) {
   return AZB_OK;
 }
 
-static_assert(sizeof(cx<double>) * (96 + 192 + 384 + 768 + 1536) + sizeof(cx<float>) * (96 + 192 + 384 + 768 + 1536) <=
+static_assert(sizeof(cx<double>) * (96 + 192 + 384 + 768 + 1536) +
+                      sizeof(cx<float>) * (96 + 192 + 384 + Row8Cfg<float, 768>::TW_END + 1536) <=
                   kScratchTwBytes,
               "twiddle tables must fit their region of the scratch array");
 
