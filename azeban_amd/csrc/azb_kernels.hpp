@@ -129,7 +129,10 @@ __global__ __launch_bounds__(TS * L / LenPlan<L>::E, (sizeof(T) == 4 ? 4 : (L ==
   constexpr int NT = TS * L / E;
   // exchange in one barrier pair unless the interleaved buffer would exceed 64 KB
   constexpr bool SPLIT = TS * L * 2 * (int)sizeof(T) > 65536;
-  using F = BlockFFT2<T, L, TS, NT, DIR, SPLIT, 1, MAP_STD, MAP_STD>;
+  // FP32 at L = 768 in the RHS: twiddles from the plan's per-pass table (the
+  // launcher points a.tw at it), one base address per butterfly and 16-byte loads
+  constexpr bool PT = sizeof(T) == 4 && L == 768 && MODE != COL_FULL;
+  using F = BlockFFT2<T, L, TS, NT, DIR, SPLIT, 1, MAP_STD, MAP_STD, false, 1, 0u, 0, 0u, void, PT>;
   constexpr int R0 = first_radix(PP{});
   constexpr int RL = last_radix(PP{});
   __shared__ T lds[ExchangeSize<T, L, TS, SPLIT>::words];
@@ -702,6 +705,20 @@ struct Row8Cfg {
   static constexpr int TW_INV = L, TW_FWD = L + pass_table_entries(typename Row8Plan<L>::Inv{});
   static constexpr int TW_END = TW_FWD + pass_table_entries(typename Row8Plan<L>::Fwd{});
 };
+
+// Per-pass twiddle tables of the FP32 768-point column passes, after the row
+// kernel's: for each radix plan V (ColPlan<768, V>) the forward table, then the
+// inverse one.
+constexpr int col_ptab_size(int V) {
+  return V == 0 ? pass_table_entries(ColPlan<768, 0>::P{})
+                : (V == 1 ? pass_table_entries(ColPlan<768, 1>::P{}) : pass_table_entries(ColPlan<768, 2>::P{}));
+}
+constexpr int col_ptab_offset(int V, int DIR) {
+  int off = Row8Cfg<float, 768>::TW_END;
+  for (int v = 0; v < V; ++v) off += 2 * col_ptab_size(v);
+  return off + (DIR > 0 ? col_ptab_size(V) : 0);
+}
+constexpr int kTw768F32End = col_ptab_offset(2, +1) + col_ptab_size(2);
 
 template <typename T, int L>
 __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3, 3))) void euler_row8_kernel(RowArgs a) {

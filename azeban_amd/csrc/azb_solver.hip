@@ -81,7 +81,7 @@ std::vector<cx<T>> host_twiddles(int L) {
 // table, followed for the FP32 768-point row kernel by its per-pass tables.
 template <typename T>
 size_t twiddle_entries(int L) {
-  if (sizeof(T) == 4 && L == 768) return Row8Cfg<float, 768>::TW_END;
+  if (sizeof(T) == 4 && L == 768) return kTw768F32End;
   return (size_t)L;
 }
 
@@ -94,6 +94,12 @@ std::vector<cx<T>> host_tables(int L) {
       using C = Row8Cfg<float, 768>;
       fill_pass_table(h.data() + C::TW_INV, h.data(), L, +1, Row8Plan<768>::Inv{});
       fill_pass_table(h.data() + C::TW_FWD, h.data(), L, -1, Row8Plan<768>::Fwd{});
+      fill_pass_table(h.data() + col_ptab_offset(0, -1), h.data(), L, -1, ColPlan<768, 0>::P{});
+      fill_pass_table(h.data() + col_ptab_offset(0, +1), h.data(), L, +1, ColPlan<768, 0>::P{});
+      fill_pass_table(h.data() + col_ptab_offset(1, -1), h.data(), L, -1, ColPlan<768, 1>::P{});
+      fill_pass_table(h.data() + col_ptab_offset(1, +1), h.data(), L, +1, ColPlan<768, 1>::P{});
+      fill_pass_table(h.data() + col_ptab_offset(2, -1), h.data(), L, -1, ColPlan<768, 2>::P{});
+      fill_pass_table(h.data() + col_ptab_offset(2, +1), h.data(), L, +1, ColPlan<768, 2>::P{});
     }
   }
   return h;
@@ -116,6 +122,8 @@ int make_twiddles(int L, void** out) {
 template <typename Tin, typename Tout, typename T, int L, int TS, int DIR, int MODE, int V = 0>
 void launch_col_TS(ColArgs a, long batches, hipStream_t st) {
   constexpr int E = LenPlan<L>::E;
+  // the FP32 768-point passes of the RHS read their plan's per-pass table
+  if constexpr (sizeof(T) == 4 && L == 768 && MODE != COL_FULL) a.tw = (const cx<T>*)a.tw + col_ptab_offset(V, DIR);
   a.ktiles = (a.nk + TS - 1) / TS;
   const long blocks = batches * a.nq * a.ktiles;
   hipLaunchKernelGGL((col_fft_kernel<Tin, Tout, T, L, TS, DIR, MODE, V>), dim3((unsigned)blocks),
@@ -542,7 +550,7 @@ int scratch_base(unsigned char** out) {
 }
 
 static_assert(sizeof(cx<double>) * (96 + 192 + 384 + 768 + 1536) +
-                      sizeof(cx<float>) * (96 + 192 + 384 + Row8Cfg<float, 768>::TW_END + 1536) <=
+                      sizeof(cx<float>) * (96 + 192 + 384 + kTw768F32End + 1536) <=
                   kScratchTwBytes,
               "twiddle tables must fit their region of the scratch array");
 
