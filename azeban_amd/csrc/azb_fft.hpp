@@ -161,14 +161,15 @@ struct DFT<2, DIR, T> {
 template <int DIR, typename T>
 struct DFT<3, DIR, T> {
   __device__ __forceinline__ static void run(cx<T>* v) {
-    const T c = T(0.86602540378443864676372317075293618);  // sin(2 pi / 3)
+    // c = DIR sin(2 pi / 3); v1, v2 = t2 +- i c d with the products fused into the
+    // sums (two operations fewer than forming i c d first)
+    const T c = T(DIR) * T(0.86602540378443864676372317075293618);
     cx<T> t1 = v[1] + v[2];
     cx<T> t2 = mk(v[0].x - T(0.5) * t1.x, v[0].y - T(0.5) * t1.y);
     cx<T> d = v[1] - v[2];
-    cx<T> t3 = mul_i<DIR>(cscale(d, c));
     v[0] = v[0] + t1;
-    v[1] = t2 + t3;
-    v[2] = t2 - t3;
+    v[1] = mk(fma(-c, d.y, t2.x), fma(c, d.x, t2.y));
+    v[2] = mk(fma(c, d.y, t2.x), fma(-c, d.x, t2.y));
   }
 };
 
@@ -192,18 +193,19 @@ struct DFT<8, DIR, T> {
     cx<T> o[4] = {v[1], v[3], v[5], v[7]};
     DFT<4, DIR, T>::run(e);
     DFT<4, DIR, T>::run(o);
-    // W8^1 = h(1 + DIR i), W8^2 = DIR i, W8^3 = h(-1 + DIR i)
-    cx<T> o1 = cscale(o[1] + mul_i<DIR>(o[1]), h);
+    // W8^1 = h(1 + DIR i), W8^2 = DIR i, W8^3 = h(-1 + DIR i); the factor h is
+    // fused into the output sums
+    cx<T> s1 = o[1] + mul_i<DIR>(o[1]);
     cx<T> o2 = mul_i<DIR>(o[2]);
-    cx<T> o3 = cscale(mul_i<DIR>(o[3]) - o[3], h);
+    cx<T> s3 = mul_i<DIR>(o[3]) - o[3];
     v[0] = e[0] + o[0];
     v[4] = e[0] - o[0];
-    v[1] = e[1] + o1;
-    v[5] = e[1] - o1;
+    v[1] = mk(fma(h, s1.x, e[1].x), fma(h, s1.y, e[1].y));
+    v[5] = mk(fma(-h, s1.x, e[1].x), fma(-h, s1.y, e[1].y));
     v[2] = e[2] + o2;
     v[6] = e[2] - o2;
-    v[3] = e[3] + o3;
-    v[7] = e[3] - o3;
+    v[3] = mk(fma(h, s3.x, e[3].x), fma(h, s3.y, e[3].y));
+    v[7] = mk(fma(-h, s3.x, e[3].x), fma(-h, s3.y, e[3].y));
   }
 };
 
