@@ -244,7 +244,11 @@ struct KArgs {
   long bstride;          // component stride of B_hat
   double eps, s2, k0;    // SmoothCutoff1D with s2 = 2s
   int mode;
-  const DevState* st;    // one per sample
+  DevState* st;          // one per sample
+  // fold = 1 (2D): the max|u| reduction of this RHS runs here instead of in a kernel
+  // of its own: u_max = max(bins[slot]) / Md, at stage 1 dt = min(C / (Nd u_max), max_dt)
+  int fold, slot;
+  double Md, C, Nd;
 };
 
 __device__ __forceinline__ double sv_eval(double k, double eps, double s2, double k0) {
@@ -269,8 +273,27 @@ __global__ __launch_bounds__(256) void kspace_kernel(KArgs a) {
   // grid.y = sample; samples are stacked with strides DIM*nmodes (state) and
   // 3(DIM-1)*bstride (B_hat)
   const long sample = blockIdx.y;
-  const DevState* st = a.st + sample;
+  DevState* st = a.st + sample;
   if (st->done) return;
+  // stage step: from the device state, or (fold) from the row kernel's max|u| bins,
+  // which every wave reduces itself at stage 1 (the first workgroup of a sample in
+  // every mode, to store u_max); the same operations as finalize_umax
+  double dt;
+  if (a.fold) {
+    const bool first = blockIdx.x == 0;
+    double um = 0.0;
+    if (a.mode == K_RK1 || first) {
+      const double v = __longlong_as_double((long long)st->umax_bins[a.slot][threadIdx.x & (kBins - 1)]);
+      um = wave_max<double>(v) / a.Md;
+    }
+    dt = a.mode == K_RK1 ? fmin(a.C / (a.Nd * um), st->max_dt) : st->dt;
+    if (first && threadIdx.x == 0) {
+      st->u_max = um;
+      if (a.mode == K_RK1) st->dt = dt;
+    }
+  } else {
+    dt = st->dt;
+  }
   const long idx = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= a.nmodes) return;
   const double twopi = 2.0 * 3.14159265358979323846;
@@ -337,7 +360,6 @@ __global__ __launch_bounds__(256) void kspace_kernel(KArgs a) {
     }
   }
   const double nu = sv_eval(sqrt(absk2), a.eps, a.s2, a.k0);
-  const double dt = st->dt;
 #pragma unroll
   for (int c = 0; c < DIM; ++c) {
     const long o = (sample * DIM + c) * a.nmodes + idx;

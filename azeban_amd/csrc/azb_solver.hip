@@ -395,9 +395,18 @@ int enqueue_rhs_T(azb_solver* s, const double2* u_in, const double2* u0, double2
     }
   }
   const double Md = std::pow((double)M, s->dim);
-  hipLaunchKernelGGL(finalize_umax, dim3((unsigned)S), dim3(64), 0, st, s->st, slot, kmode == K_RK1 ? 1 : 0,
-                     Md, C, (double)N);
   KArgs k{};
+  // 2D: the max|u| reduction runs inside the k-space kernel (a launch less per RHS
+  // where launches are short); 3D keeps it in its own one-workgroup kernel rather
+  // than have each of a million waves re-read the bins at stage 1
+  k.fold = s->dim == 2 ? 1 : 0;
+  k.slot = slot;
+  k.Md = Md;
+  k.C = C;
+  k.Nd = (double)N;
+  if (!k.fold)
+    hipLaunchKernelGGL(finalize_umax, dim3((unsigned)S), dim3(64), 0, st, s->st, slot, kmode == K_RK1 ? 1 : 0,
+                       Md, C, (double)N);
   k.bh = s->Y;
   k.u_in = u_in;
   k.u0 = u0;
