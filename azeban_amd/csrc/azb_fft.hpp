@@ -472,11 +472,6 @@ namespace azb {
 //   MAP_SEQ1: MAP_SEQ with G = 1 whatever the transform's G: consecutive threads own
 //             consecutive butterflies of one sequence (a radix-L/64 pass puts each
 //             sequence in exactly one wave, butterfly j in lane j)
-//   MAP_MIRROR: one butterfly per thread for a sequence of NB = L/R butterflies
-//             (NB a multiple of 64) spread over NB/64 waves so that butterfly j and
-//             its mirror NB - j sit in lanes l and l ^ 32 of one wave: lanes 0..31
-//             of wave w hold 32 w + l, lanes 32..63 the mirrors (wave 0 pairs the
-//             two self-mirrored butterflies 0 and NB/2 in lanes 0 and 32)
 //   MAP_WAVE: wave w owns sequence w (NT = 64 TS and L = 64 E), lane l its
 //             butterflies l + 64 b
 //   MAP_HALF: half-wave h (32 consecutive threads) owns sequence h (NT = 32 TS and
@@ -489,7 +484,7 @@ namespace azb {
 // pass: the LDS exchange writes with the mapping of the pass that produced the
 // data and reads with the mapping of the next pass.
 // ---------------------------------------------------------------------------
-enum { MAP_STD = 0, MAP_SEQ = 1, MAP_SEQ1 = 3, MAP_MIRROR = 4, MAP_WAVE = 5, MAP_HALF = 6 };
+enum { MAP_STD = 0, MAP_SEQ = 1, MAP_SEQ1 = 3, MAP_WAVE = 5, MAP_HALF = 6 };
 
 // Index arithmetic is unsigned throughout: signed division and modulo by a power of
 // two cost a sign fix-up each, and these run once per element of every exchange.
@@ -519,15 +514,7 @@ __device__ __forceinline__ void bmap(unsigned t, unsigned b, unsigned& s, unsign
     j = u % (unsigned)(L / R);
     s = (u / (unsigned)(L / R)) * GG + gg;
   } else {
-    static_assert(MAP == MAP_MIRROR, "unknown mapping");
-    constexpr unsigned NB = L / R;
-    const unsigned u = t + NT * b;
-    s = u / NB;
-    const unsigned i = u % NB, w = i / 64, l = i % 64;
-    if (l < 32)
-      j = 32 * w + l;
-    else
-      j = (w == 0 && l == 32) ? NB / 2 : NB - (32 * w + l - 32);
+    static_assert(MAP != MAP, "unknown mapping");
   }
 }
 

@@ -38,8 +38,13 @@ void launch_row_L(const RowArgs& a, long nsamples, hipStream_t st, int pdiv) {
       launch_row4_P<T, L, P1>(a, nsamples, st);
     else
       launch_row4_P<T, L, P0>(a, nsamples, st);
+  } else if constexpr (L == 1536) {
+    // 2D at L = 1536: one line pair per 192-thread workgroup, 24 values per thread
+    const long blocks = a.nlines / 2 * nsamples;
+    hipLaunchKernelGGL((euler_row2w_kernel<T>), dim3((unsigned)blocks), dim3(Row2wCfg::NT), 0, st, a);
+    (void)pdiv;
   } else {
-    // one shape: max(768/L, 1) line pairs, 192 threads (384 at L = 1536)
+    // one shape: 768/L line pairs, 192 threads
     const long blocks = a.nlines / (2 * Row2Cfg<T, L>::P) * nsamples;
     hipLaunchKernelGGL((euler_row2_kernel<T, L>), dim3((unsigned)blocks), dim3(Row2Cfg<T, L>::NT), 0, st, a);
     (void)pdiv;

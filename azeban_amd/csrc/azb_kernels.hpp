@@ -540,8 +540,8 @@ __device__ __forceinline__ void row_first_pass(cx<T>* v, const cx<T>* __restrict
 
 // Separation of a packed forward transform Z = A + iB (A, B spectra of two real
 // lines) after a last pass of radix R (12 or 24) with butterfly j of a sequence in
-// lane j of its wave (NB = L/R dividing 64) or in the MAP_MIRROR layout (NB a
-// multiple of 64).  The thread holds Z[j + NB r], r < R; the mirror of position
+// lane j of its wave or half-wave (NB = L/R dividing 64).  The thread holds
+// Z[j + NB r], r < R; the mirror of position
 // j + NB r is (NB - j) + NB (R - 1 - r), index R - 1 - r of the lane holding
 // butterfly NB - j (for j = 0: index R - r of this lane).  Stores A[k], B[k] for the
 // k <= N/2 = (R/3) NB this lane owns (r < R/3, and r = R/3 for j = 0) as contiguous
@@ -550,10 +550,9 @@ template <typename T, int NB, int R = 12>
 __device__ __forceinline__ void row_separate_store(const cx<T>* v, unsigned lane, unsigned j, vec2_t<T>* oa,
                                                    vec2_t<T>* ob) {
   constexpr int R3 = R / 3;
-  // NB <= 64: butterfly j in lane j of a wave per sequence; NB > 64: MAP_MIRROR
-  // (mirror in lane ^ 32, the self-mirrored 0 and NB/2 in lanes 0 and 32)
-  const int mlane = NB <= 64 ? (int)(lane - j + (NB - j) % NB)
-                             : (int)((j == 0 || j == NB / 2) ? lane : (lane ^ 32u));
+  // butterfly j of a sequence in lane j of its wave (or half-wave)
+  static_assert(64 % NB == 0, "a sequence's butterflies within one wave");
+  const int mlane = (int)(lane - j + (NB - j) % NB);
   cx<T> f[R3];
 #pragma unroll
   for (int q = 0; q < R3; ++q) {
@@ -990,36 +989,31 @@ template <> struct Row2Plan<768> { using Inv = Plan<12, 4, 4, 4>; using Fwd = Pl
 template <> struct Row2Plan<384> { using Inv = Plan<12, 8, 4>; using Fwd = Plan<4, 4, 2, 12>; };
 template <> struct Row2Plan<192> { using Inv = Plan<12, 4, 4>; using Fwd = Plan<4, 4, 12>; };
 template <> struct Row2Plan<96> { using Inv = Plan<12, 2, 4>; using Fwd = Plan<4, 2, 12>; };
-template <> struct Row2Plan<1536> { using Inv = Plan<12, 8, 4, 4>; using Fwd = Plan<4, 4, 4, 2, 12>; };
 
 template <typename T, int L>
 struct Row2Cfg {
-  static constexpr int P = L <= 768 ? 768 / L : 1;
+  static_assert(L <= 768, "L = 1536 has its own kernel (euler_row2w_kernel)");
+  static constexpr int P = 768 / L;
   static constexpr int NT = P * L / 4;
   static constexpr int TSI = 2 * P, TSF = 3 * P;
   static constexpr int NB = L / 12;
   static constexpr int SOFF = 11;
-  // FP64 at L = 1536: the forward exchange goes in two halves (real, imaginary) so
-  // that three workgroups fit a CU and one launch of a single 1024^2 sample
-  // (768 workgroups) is resident at once
-  static constexpr bool FSPLIT = sizeof(T) == 8 && L == 1536;
-  static constexpr int WPE = FSPLIT ? 5 : 1;
   static constexpr int XI = 2 * ExLayout<T, L, TSI, true, SOFF>::elems;
-  static constexpr int XF = (FSPLIT ? 1 : 2) * ExLayout<T, L, TSF, true, SOFF>::elems;
+  static constexpr int XF = 2 * ExLayout<T, L, TSF, true, SOFF>::elems;
   static constexpr int XWORDS = XI > XF ? XI : XF;
 };
 
 template <typename T, int L>
-__global__ __launch_bounds__((Row2Cfg<T, L>::NT)) __attribute__((amdgpu_waves_per_eu(Row2Cfg<T, L>::WPE))) void euler_row2_kernel(RowArgs a) {
+__global__ __launch_bounds__((Row2Cfg<T, L>::NT)) void euler_row2_kernel(RowArgs a) {
   using C = Row2Cfg<T, L>;
   using PL = Row2Plan<L>;
   constexpr int NT = C::NT, P = C::P, NB = C::NB, TSI = C::TSI, TSF = C::TSF;
   using FI = BlockFFT2<T, L, TSI, NT, +1, false, 2, MAP_STD, MAP_SEQ, true, C::SOFF>;
-  constexpr int MAPL = NB <= 64 ? MAP_SEQ1 : MAP_MIRROR;
-  using FF = BlockFFT2<T, L, TSF, NT, -1, C::FSPLIT, 3, MAP_SEQ, MAPL, true, C::SOFF>;
+  constexpr int MAPL = MAP_SEQ1;
+  using FF = BlockFFT2<T, L, TSF, NT, -1, false, 3, MAP_SEQ, MAPL, true, C::SOFF>;
   using LY = ExLayout<T, L, TSF, true, C::SOFF>;
   static_assert(FI::E == 8 && FF::E == 12, "thread layout");
-  static_assert((64 % NB == 0 || NB % 64 == 0) && TSI * NB % 64 == 0, "whole waves in the radix-12 passes");
+  static_assert(64 % NB == 0 && TSI * NB % 64 == 0, "whole waves in the radix-12 passes");
   static_assert(first_radix(typename PL::Inv{}) == 12 && last_radix(typename PL::Inv{}) == 4 &&
                     first_radix(typename PL::Fwd{}) == 4 && last_radix(typename PL::Fwd{}) == 12,
                 "plans must start/end on the radices the mappings assume");
@@ -1080,6 +1074,110 @@ __global__ __launch_bounds__((Row2Cfg<T, L>::NT)) __attribute__((amdgpu_waves_pe
     const long la = line0 + 2 * pair, lb = la + 1;
     row_separate_store<T, NB>(w, tt & 63, j, (vec2_t<T>*)(out + ((long)m * a.nlines + la) * a.ls),
                               (vec2_t<T>*)(out + ((long)m * a.nlines + lb) * a.ls));
+  }
+
+  lmax = wave_max<T>(lmax);
+  if ((t & 63) == 0) red[t >> 6] = lmax;
+  __syncthreads();
+  if (t == 0) {
+    double mx = 0.0;
+    for (int q = 0; q < NT / 64; ++q) mx = fmax(mx, red[q]);
+    atomicMax(&st->umax_bins[a.slot][blockIdx.x & (kBins - 1)],
+              (unsigned long long)__double_as_longlong(mx));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 2D row kernel at L = 1536 (N = 1024) with 24 values per thread in the forward.
+// One line pair per workgroup of 192 threads.  Inverse (plan 24.8.8): the radix-24
+// first pass from HBM on 128 threads (butterfly j of sequence s, NB = 64), written
+// to LDS, then two radix-8 passes on all 192 threads with 16 values each; the last
+// leaves u1 + i u2 of both lines at the same eight positions t + 192 r in thread t.
+// Forward (plan 8.8.24, three sequences B11, B12, B22 packed line a + i line b, 24
+// values per thread, exchanges in real/imaginary halves): the radix-8 middle pass
+// keeps each sequence in its own wave, so the exchange into the radix-24 pass (one
+// sequence per wave, butterfly j in lane j) needs no workgroup barrier, and the
+// separation of each packed spectrum stays within a wave.  Two exchanges per
+// transform against three (inverse) and four (forward) with 8 and 12 values per
+// thread: fewer LDS round trips and barriers in a launch whose 768 workgroups are
+// all resident at once, where a workgroup's own chain sets the time.
+// ---------------------------------------------------------------------------
+struct Row2wCfg {
+  static constexpr int L = 1536;
+  static constexpr int NT = 192;
+  static constexpr int NB = L / 24;  // radix-24 butterflies per sequence
+  static constexpr int SOFF = 11;
+  using LI = ExLayout<double, L, 2, true, SOFF>;
+  using LF = ExLayout<double, L, 3, true, SOFF>;
+  using PI = Plan<24, 8, 8>;
+  using PF = Plan<8, 8, 24>;
+};
+
+template <typename T>
+__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void euler_row2w_kernel(RowArgs a) {
+  using C = Row2wCfg;
+  constexpr int L = C::L, NT = C::NT, NB = C::NB;
+  // forward exchange halves in FP64 (8-byte slots) keep the buffer at the inverse's
+  using FI = BlockFFT2<T, L, 2, NT, +1, false, 2, MAP_STD, MAP_SEQ, true, C::SOFF>;
+  using FF = BlockFFT2<T, L, 3, NT, -1, sizeof(T) == 8, 3, MAP_SEQ, MAP_SEQ1, true, C::SOFF, 0b010u>;
+  using LY = ExLayout<T, L, 2, true, C::SOFF>;
+  static_assert(FI::E == 16 && FF::E == 24, "thread layout");
+  constexpr int XWORDS = FI::lds_words > FF::lds_words ? FI::lds_words : FF::lds_words;
+  __shared__ T X[XWORDS];
+  __shared__ double red[NT / 64];
+  const long bps = a.nlines / 2;  // workgroups per sample
+  const long sample = blockIdx.x / bps;
+  DevState* st = a.st + sample;
+  if (st->done) return;
+
+  const int t = threadIdx.x;
+  const long la = (blockIdx.x % bps) * 2, lb = la + 1;
+  const cx<T>* __restrict__ in = (const cx<T>*)a.in + sample * a.in_ss;
+  cx<T>* __restrict__ out = (cx<T>*)a.out + sample * a.out_ss;
+  const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
+  const T nrm = T(a.norm);
+
+  cx<T> v[24];
+  {
+    unsigned tt = t;
+    asm volatile("" : "+v"(tt));
+    if (tt < (unsigned)(2 * NB)) {
+      const unsigned s = tt / NB, j = tt % NB;  // sequence s: u1 + i u2 of line la + s
+      const long l = la + s;
+      row_first_pass<T, NB, 24>(v, in + l * a.ls, in + (a.nlines + l) * a.ls, j);
+      // Stockham pass-0 outputs of butterfly j sit at positions 24 j + r
+#pragma unroll
+      for (int r = 0; r < 24; ++r) {
+        const unsigned idx = LY::at(s, 24 * j + r);
+        X[2 * idx] = v[r].x;
+        X[2 * idx + 1] = v[r].y;
+      }
+    }
+  }
+  __syncthreads();
+  FI::run_after_first(v, t, X, tw, typename C::PI{});
+
+  // v[g*8 + r]: u1 + i u2 of line g at positions t + 192 r
+  T lm = T(0);
+  cx<T> w[24];
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+    const cx<T> pa = v[r], pb = v[8 + r];
+    lm = fmax(lm, fmax(fmax(fabs(pa.x), fabs(pa.y)), fmax(fabs(pb.x), fabs(pb.y))));
+    const T a1 = nrm * pa.x, a2 = nrm * pa.y, b1 = nrm * pb.x, b2 = nrm * pb.y;
+    w[r] = mk(a1 * a1, b1 * b1);       // B11
+    w[8 + r] = mk(a1 * a2, b1 * b2);   // B12
+    w[16 + r] = mk(a2 * a2, b2 * b2);  // B22
+  }
+  double lmax = (double)lm;
+  asm volatile("" : "+v"(lmax));
+  FF::run(w, t, X, tw, typename C::PF{});
+  {
+    unsigned tt = t;
+    asm volatile("" : "+v"(tt));
+    const unsigned m = tt / NB, j = tt % NB;
+    row_separate_store<T, NB, 24>(w, tt & 63, j, (vec2_t<T>*)(out + ((long)m * a.nlines + la) * a.ls),
+                                  (vec2_t<T>*)(out + ((long)m * a.nlines + lb) * a.ls));
   }
 
   lmax = wave_max<T>(lmax);
