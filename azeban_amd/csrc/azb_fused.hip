@@ -43,6 +43,11 @@ void launch_row_L(const RowArgs& a, long nsamples, hipStream_t st, int pdiv) {
     const long blocks = a.nlines / 2 * nsamples;
     hipLaunchKernelGGL((euler_row2w_kernel<T>), dim3((unsigned)blocks), dim3(Row2wCfg::NT), 0, st, a);
     (void)pdiv;
+  } else if constexpr (L == 768) {
+    // 2D at L = 768: two line pairs per 192-thread workgroup, the 3D forward
+    const long blocks = a.nlines / 4 * nsamples;
+    hipLaunchKernelGGL((euler_row2p_kernel<T>), dim3((unsigned)blocks), dim3(Row2pCfg::NT), 0, st, a);
+    (void)pdiv;
   } else {
     // one shape: 768/L line pairs, 192 threads
     const long blocks = a.nlines / (2 * Row2Cfg<T, L>::P) * nsamples;
@@ -58,8 +63,8 @@ int launch_row_T(int L, const RowArgs& a, long nsamples, hipStream_t st, int pdi
     case 192: launch_row_L<T, 192, DIM>(a, nsamples, st, pdiv); return 0;
     case 384: launch_row_L<T, 384, DIM>(a, nsamples, st, pdiv); return 0;
     case 768:
-      if constexpr (DIM == 3 && sizeof(T) == 4) {
-        if (a.nlines % (2 * Row8Cfg<T, 768>::P) != 0) return 2;
+      if constexpr ((DIM == 3 && sizeof(T) == 4) || DIM == 2) {
+        if (a.nlines % 4 != 0) return 2;  // two line pairs per workgroup
       }
       if (a.nlines % 2 != 0) return 2;
       launch_row_L<T, 768, DIM>(a, nsamples, st, pdiv);

@@ -973,7 +973,7 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 }
 
 // ---------------------------------------------------------------------------
-// 2D row kernel (L <= 768), the 3D kernel's layout with one product round.
+// 2D row kernel (L <= 384), the 3D kernel's layout with one product round.
 // Per workgroup: P = 768/L line pairs, 192 threads.  Inverse: 2P packed sequences
 // (u1 + i u2 of each line, plan 12 ...): the radix-12 first pass (loaded from HBM)
 // runs on the first 2P * L/12 = 128 threads, one butterfly each, the later passes
@@ -985,14 +985,13 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 // ---------------------------------------------------------------------------
 template <int L>
 struct Row2Plan;
-template <> struct Row2Plan<768> { using Inv = Plan<12, 4, 4, 4>; using Fwd = Plan<4, 4, 4, 12>; };
 template <> struct Row2Plan<384> { using Inv = Plan<12, 8, 4>; using Fwd = Plan<4, 4, 2, 12>; };
 template <> struct Row2Plan<192> { using Inv = Plan<12, 4, 4>; using Fwd = Plan<4, 4, 12>; };
 template <> struct Row2Plan<96> { using Inv = Plan<12, 2, 4>; using Fwd = Plan<4, 2, 12>; };
 
 template <typename T, int L>
 struct Row2Cfg {
-  static_assert(L <= 768, "L = 1536 has its own kernel (euler_row2w_kernel)");
+  static_assert(L <= 384, "L = 768 and 1536 have kernels of their own (euler_row2p_kernel, euler_row2w_kernel)");
   static constexpr int P = 768 / L;
   static constexpr int NT = P * L / 4;
   static constexpr int TSI = 2 * P, TSF = 3 * P;
@@ -1176,6 +1175,117 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void e
     unsigned tt = t;
     asm volatile("" : "+v"(tt));
     const unsigned m = tt / NB, j = tt % NB;
+    row_separate_store<T, NB, 24>(w, tt & 63, j, (vec2_t<T>*)(out + ((long)m * a.nlines + la) * a.ls),
+                                  (vec2_t<T>*)(out + ((long)m * a.nlines + lb) * a.ls));
+  }
+
+  lmax = wave_max<T>(lmax);
+  if ((t & 63) == 0) red[t >> 6] = lmax;
+  __syncthreads();
+  if (t == 0) {
+    double mx = 0.0;
+    for (int q = 0; q < NT / 64; ++q) mx = fmax(mx, red[q]);
+    atomicMax(&st->umax_bins[a.slot][blockIdx.x & (kBins - 1)],
+              (unsigned long long)__double_as_longlong(mx));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 2D row kernel at L = 768 (N = 512) with two line pairs per 192-thread workgroup
+// and the 3D kernels' forward: six sequences (B11, B12, B22 of each pair, packed
+// line a + i line b), 24 values per thread, plan 4.8.24 with the radix-8 pass in
+// half-waves; FP64 as euler_row6_kernel (exchanges in real/imaginary halves,
+// swizzled second exchange), FP32 as euler_row8_kernel (per-pass twiddle tables).
+// Inverse (plan 24.8.4): the radix-24 first pass from HBM on 128 threads (four
+// sequences u1 + i u2 of lines 0..3, butterfly j in lane j of a half-wave), written
+// to LDS, then 16 values per thread on all 192 threads; the radix-4 last pass
+// leaves all four sequences at positions t + 192 r in thread t, where the
+// products are formed.  Two exchanges per transform instead of three.
+// ---------------------------------------------------------------------------
+struct Row2pCfg {
+  static constexpr int L = 768;
+  static constexpr int NT = 192;
+  static constexpr int NB = L / 24;  // radix-24 butterflies per sequence
+  static constexpr int SOFFI = 11;
+  using PI = Plan<24, 8, 4>;
+  using PF = Plan<4, 8, 24>;
+};
+
+template <typename T>
+__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void euler_row2p_kernel(RowArgs a) {
+  using C = Row2pCfg;
+  constexpr int L = C::L, NT = C::NT, NB = C::NB;
+  constexpr bool F32 = sizeof(T) == 4;
+  using FI = BlockFFT2<T, L, 4, NT, +1, false, 4, MAP_STD, MAP_SEQ, true, C::SOFFI, 0u, 0, 0u, void, F32>;
+  using FF = std::conditional_t<F32,
+                                BlockFFT2<T, L, 6, NT, -1, false, 6, MAP_SEQ, MAP_SEQ1, true, 5, 0u, 0, 0b010u, void, true>,
+                                BlockFFT2<T, L, 6, NT, -1, true, 6, MAP_SEQ, MAP_SEQ1, true, Row6Cfg::SOFFF, 0u, 32,
+                                          0b010u, typename Row6Cfg::FwdLay>>;
+  using LY = ExLayout<T, L, 4, true, C::SOFFI>;
+  static_assert(FI::E == 16 && FF::E == 24, "thread layout");
+  constexpr int XWORDS = FI::lds_words > FF::lds_words ? FI::lds_words : FF::lds_words;
+  __shared__ T X[XWORDS];
+  __shared__ double red[NT / 64];
+  const long bps = a.nlines / 4;  // workgroups per sample
+  const long sample = blockIdx.x / bps;
+  DevState* st = a.st + sample;
+  if (st->done) return;
+
+  const int t = threadIdx.x;
+  const long line0 = (blockIdx.x % bps) * 4;
+  const cx<T>* __restrict__ in = (const cx<T>*)a.in + sample * a.in_ss;
+  cx<T>* __restrict__ out = (cx<T>*)a.out + sample * a.out_ss;
+  const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
+  // FP32: per-pass twiddle tables (the 3D FP32 row kernel's, same plans)
+  const cx<T>* twi = F32 ? tw + Row8Cfg<float, 768>::TW_INV : tw;
+  const cx<T>* twf = F32 ? tw + Row8Cfg<float, 768>::TW_FWD : tw;
+  const T nrm = T(a.norm);
+
+  cx<T> v[24];
+  {
+    unsigned tt = t;
+    asm volatile("" : "+v"(tt));
+    if (tt < (unsigned)(4 * NB)) {
+      const unsigned s = tt / NB, j = tt % NB;  // sequence s: u1 + i u2 of line line0 + s
+      const long l = line0 + s;
+      row_first_pass<T, NB, 24>(v, in + l * a.ls, in + (a.nlines + l) * a.ls, j);
+      // Stockham pass-0 outputs of butterfly j sit at positions 24 j + r
+#pragma unroll
+      for (int r = 0; r < 24; ++r) {
+        const unsigned idx = LY::at(s, 24 * j + r);
+        X[2 * idx] = v[r].x;
+        X[2 * idx + 1] = v[r].y;
+      }
+    }
+  }
+  __syncthreads();
+  FI::run_after_first(v, t, X, twi, typename C::PI{});
+
+  // v[g*4 + r]: u1 + i u2 of line g (0..3) at positions t + 192 r; products of pair
+  // p (lines 2p, 2p + 1) into sequences 3p + m, m = B11, B12, B22
+  T lm = T(0);
+  cx<T> w[24];
+#pragma unroll
+  for (int p = 0; p < 2; ++p) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const cx<T> pa = v[(2 * p) * 4 + r], pb = v[(2 * p + 1) * 4 + r];
+      lm = fmax(lm, fmax(fmax(fabs(pa.x), fabs(pa.y)), fmax(fabs(pb.x), fabs(pb.y))));
+      const T a1 = nrm * pa.x, a2 = nrm * pa.y, b1 = nrm * pb.x, b2 = nrm * pb.y;
+      w[(3 * p) * 4 + r] = mk(a1 * a1, b1 * b1);      // B11
+      w[(3 * p + 1) * 4 + r] = mk(a1 * a2, b1 * b2);  // B12
+      w[(3 * p + 2) * 4 + r] = mk(a2 * a2, b2 * b2);  // B22
+    }
+  }
+  double lmax = (double)lm;
+  asm volatile("" : "+v"(lmax));
+  FF::run(w, t, X, twf, typename C::PF{});
+  {
+    unsigned tt = t;
+    asm volatile("" : "+v"(tt));
+    const unsigned s = tt / NB, j = tt % NB;
+    const unsigned pair = s / 3, m = s % 3;
+    const long la = line0 + 2 * pair, lb = la + 1;
     row_separate_store<T, NB, 24>(w, tt & 63, j, (vec2_t<T>*)(out + ((long)m * a.nlines + la) * a.ls),
                                   (vec2_t<T>*)(out + ((long)m * a.nlines + lb) * a.ls));
   }
