@@ -301,7 +301,7 @@ class Server:
 
 
 def serve_one_at_a_time(requests: list[Request], C_: float = 0.2, chunk: int = 16) -> dict:
-    """The baseline the server is judged against: the same requests in arrival order,
+    """The baseline the server is compared with: the same requests in arrival order,
     each solved alone (one IncompressibleEuler per grid size, created before the clock
     starts and reused) once it has arrived and the previous one has finished."""
     from .ensemble import solve_one
