@@ -33,6 +33,12 @@ void launch_row_L(const RowArgs& a, long nsamples, hipStream_t st, int pdiv) {
     const long blocks = a.nlines / 2 * nsamples;
     hipLaunchKernelGGL((euler_row6_kernel<T>), dim3((unsigned)blocks), dim3(Row6Cfg::NT), 0, st, a);
     (void)pdiv;
+  } else if constexpr (DIM == 3 && L == 384) {
+    // 3D at L = 384: two line pairs per workgroup with a single twelve-sequence
+    // forward round, whatever pdiv says (one plan per size)
+    const long blocks = a.nlines / 4 * nsamples;
+    hipLaunchKernelGGL((euler_row6p_kernel<T>), dim3((unsigned)blocks), dim3(Row6pCfg::NT), 0, st, a);
+    (void)pdiv;
   } else if constexpr (DIM == 3) {
     if (pdiv == 2)
       launch_row4_P<T, L, P1>(a, nsamples, st);
@@ -61,7 +67,10 @@ int launch_row_T(int L, const RowArgs& a, long nsamples, hipStream_t st, int pdi
   switch (L) {
     case 96: launch_row_L<T, 96, DIM>(a, nsamples, st, pdiv); return 0;
     case 192: launch_row_L<T, 192, DIM>(a, nsamples, st, pdiv); return 0;
-    case 384: launch_row_L<T, 384, DIM>(a, nsamples, st, pdiv); return 0;
+    case 384:
+      if (DIM == 3 && a.nlines % 4 != 0) return 2;  // two line pairs per workgroup
+      launch_row_L<T, 384, DIM>(a, nsamples, st, pdiv);
+      return 0;
     case 768:
       if constexpr ((DIM == 3 && sizeof(T) == 4) || DIM == 2) {
         if (a.nlines % 4 != 0) return 2;  // two line pairs per workgroup

@@ -1301,4 +1301,106 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void e
   }
 }
 
+// ---------------------------------------------------------------------------
+// 3D row kernel at L = 384 (N = 256) with one forward round: two line pairs per
+// 192-thread workgroup.  The inverse is the row-4 kernel's (six packed sequences,
+// 12 values per thread, plan 12.2.4.4) with its two middle passes in half-waves
+// (each sequence in its own half-wave: two of the three exchanges without a
+// workgroup barrier); all six products of a pair are formed at once and one
+// twelve-sequence forward transform (24 values per thread, plan 4.4.24, exchanges
+// in real/imaginary halves in FP64) replaces two rounds of six.  The radix-24 last
+// pass keeps butterfly j of a sequence in lane j of a 16-lane group, so the
+// separation of each packed spectrum stays within a wave.
+// ---------------------------------------------------------------------------
+struct Row6pCfg {
+  static constexpr int L = 384;
+  static constexpr int NT = 192;
+  static constexpr int NBI = L / 12, NBF = L / 24;
+  // sequence strides chosen so that four workgroups' exchange buffers fit a CU
+  static constexpr int SOFFI = 5, SOFFF = 5;
+  using PI = Plan<12, 2, 4, 4>;
+  using PF = Plan<4, 4, 24>;
+};
+
+template <typename T>
+__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3, 3))) void euler_row6p_kernel(RowArgs a) {
+  using C = Row6pCfg;
+  constexpr int L = C::L, NT = C::NT, NBI = C::NBI, NBF = C::NBF;
+  constexpr bool F64 = sizeof(T) == 8;
+  using FI = BlockFFT2<T, L, 6, NT, +1, false, 3, MAP_SEQ1, MAP_SEQ, true, C::SOFFI, 0u, 0, 0b0110u>;
+  using FF = BlockFFT2<T, L, 12, NT, -1, F64, 6, MAP_SEQ, MAP_SEQ1, true, C::SOFFF, 0u, F64 ? 32 : 0>;
+  static_assert(FI::E == 12 && FF::E == 24, "thread layout");
+  constexpr int XWORDS = FI::lds_words > FF::lds_words ? FI::lds_words : FF::lds_words;
+  static_assert(4 * (XWORDS * (int)sizeof(T) + 3 * 8) <= 163840, "four workgroups per CU");
+  __shared__ T X[XWORDS];
+  __shared__ double red[NT / 64];
+  const long bps = a.nlines / 4;  // workgroups per sample
+  const long sample = blockIdx.x / bps;
+  DevState* st = a.st + sample;
+  if (st->done) return;
+
+  const int t = threadIdx.x;
+  const long line0 = (blockIdx.x % bps) * 4;
+  const cx<T>* __restrict__ in = (const cx<T>*)a.in + sample * a.in_ss;
+  cx<T>* __restrict__ out = (cx<T>*)a.out + sample * a.out_ss;
+  const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
+  const T nrm = T(a.norm);
+
+  // first inverse pass from HBM; sequence s is type s % 3 of pair s / 3:
+  // (u1a + i u2a, u3a + i u3b, u1b + i u2b)
+  cx<T> u[12];
+  {
+    unsigned tt = t;
+    asm volatile("" : "+v"(tt));
+    const unsigned s = tt / NBI, j = tt % NBI;
+    const unsigned pair = s / 3, type = s % 3;
+    const long la = line0 + 2 * pair, lb = la + 1;
+    const long lx = type == 0 ? la : (type == 1 ? 2 * a.nlines + la : lb);
+    const long ly = type == 0 ? a.nlines + la : (type == 1 ? 2 * a.nlines + lb : a.nlines + lb);
+    row_first_pass<T, NBI>(u, in + lx * a.ls, in + ly * a.ls, j);
+  }
+  FI::run_rest(u, t, X, tw, typename C::PI{});
+  T lm = T(0);
+#pragma unroll
+  for (int e = 0; e < 12; ++e) lm = fmax(lm, fmax(fabs(u[e].x), fabs(u[e].y)));
+  double lmax = (double)lm;
+  asm volatile("" : "+v"(lmax));
+
+  // products of this thread's pair (t / 96) at its 4 positions: w[m*4 + r] =
+  // B_m (line a) + i B_m (line b), m = 0..5: B11, B21, B22, B31, B32, B33
+  cx<T> w[24];
+#pragma unroll
+  for (int r = 0; r < 4; ++r) {
+    const cx<T> p0 = u[r], p1 = u[4 + r], p2 = u[8 + r];
+    const T a1 = nrm * p0.x, a2 = nrm * p0.y, a3 = nrm * p1.x;
+    const T b1 = nrm * p2.x, b2 = nrm * p2.y, b3 = nrm * p1.y;
+    w[0 * 4 + r] = mk(a1 * a1, b1 * b1);
+    w[1 * 4 + r] = mk(a2 * a1, b2 * b1);
+    w[2 * 4 + r] = mk(a2 * a2, b2 * b2);
+    w[3 * 4 + r] = mk(a3 * a1, b3 * b1);
+    w[4 * 4 + r] = mk(a3 * a2, b3 * b2);
+    w[5 * 4 + r] = mk(a3 * a3, b3 * b3);
+  }
+  FF::run(w, t, X, tw, typename C::PF{});
+  {
+    unsigned tt = t;
+    asm volatile("" : "+v"(tt));
+    const unsigned s = tt / NBF, j = tt % NBF;
+    const unsigned pair = s / 6, m = s % 6;
+    const long la = line0 + 2 * pair, lb = la + 1;
+    row_separate_store<T, NBF, 24>(w, tt & 63, j, (vec2_t<T>*)(out + ((long)m * a.nlines + la) * a.ls),
+                                   (vec2_t<T>*)(out + ((long)m * a.nlines + lb) * a.ls));
+  }
+
+  lmax = wave_max<T>(lmax);
+  if ((t & 63) == 0) red[t >> 6] = lmax;
+  __syncthreads();
+  if (t == 0) {
+    double mx = 0.0;
+    for (int q = 0; q < NT / 64; ++q) mx = fmax(mx, red[q]);
+    atomicMax(&st->umax_bins[a.slot][blockIdx.x & (kBins - 1)],
+              (unsigned long long)__double_as_longlong(mx));
+  }
+}
+
 }  // namespace azb
