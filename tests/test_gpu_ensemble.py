@@ -48,6 +48,24 @@ def test_ensemble_matches_single_solves_3d(cuda):
     ens.close()
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32_mixed"])
+def test_ensemble_matches_single_solves_3d_256(cuda, precision):
+    """3D batches at N = 256 (two line pairs per row-kernel workgroup, samples
+    stacked in the launch) give each sample's single-solve result bit for bit."""
+    print(f"[case] 3D ensemble N=256 {precision}", flush=True)
+    t_end = 5e-3  # at least two steps: the viscous clamp caps dt at 3.2e-3
+    u0 = torch.stack([initialize(I.fractional_brownian(3, 256, seed=s)) for s in (4, 5)]).contiguous()
+    g = Group(3, 256, u0.clone(), precision)
+    ens = Ensemble([g])
+    res = ens.run(t_end, steps_per_launch=2)
+    torch.cuda.synchronize()
+    for si in range(2):
+        ref, n = solve_one(3, 256, u0[si], t_end, precision=precision)
+        assert n >= 2 and res["steps"][0][si] == n
+        assert torch.equal(g.u[si], ref), si
+    ens.close()
+
+
 def test_graph_replays_repeatedly_match_eager(cuda):
     """Replaying the captured graph several times, with eager work and host syncs
     between replays, matches enqueuing the same steps eagerly."""

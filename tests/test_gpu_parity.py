@@ -599,13 +599,17 @@ def test_simulate_until_edge_cases(cuda):
     assert n == 3 and t_small < sim.time() < 1.0
 
 
-# every FP32 row-kernel path: 2D L = 384, 768 (row-2 kernel, FSPLIT off), 1536 (the
-# unsplit FP32 forward exchange), 3D L = 96 and 192 (row-4 kernel)
-@pytest.mark.parametrize("dim,N,steps", [(2, 256, 20), (2, 512, 10), (2, 1024, 10), (3, 64, 10), (3, 128, 5)])
+# every FP32 row-kernel path: 2D L = 384 (row-2 kernel), 768 (two line pairs per
+# workgroup), 1536 (24 values per thread), 3D L = 96 and 192 (row-4 kernel), 384
+# (single forward round)
+@pytest.mark.parametrize("dim,N,steps", [(2, 256, 20), (2, 512, 10), (2, 1024, 10), (3, 64, 10), (3, 128, 5),
+                                         (3, 256, 2)])
 def test_fp32_mixed_error(cuda, dim, N, steps):
     print(f"[case] fp32_mixed dim={dim} N={N}", flush=True)
     uh = random_state(dim, N, seed=6)
     g, eq, oeq = make_pair(dim, N, "fp32_mixed")
+    if dim == 3 and N >= 256:
+        oeq = O.IncompressibleEuler(O.Grid(3, N), O.baseline_visc(N), workers=FFT_WORKERS)
     sim = A.Simulation(g, 0.2, A.SSP_RK3(g.shape_fourier(dim), eq), dev(uh))
     osim = O.Simulation(oeq, uh, 0.2)
     sim.simulate_until(10.0, max_steps=steps)
