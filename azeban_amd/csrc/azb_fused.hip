@@ -15,11 +15,11 @@ void launch_row4_P(const RowArgs& a, long nsamples, hipStream_t st) {
   hipLaunchKernelGGL((euler_row4_kernel<T, L, P>), dim3((unsigned)blocks), dim3(Row4Cfg<T, L, P>::NT), 0, st, a);
 }
 
-// pdiv in {1, 2}: P0 = 1536/L line pairs per workgroup, or half of that
+// pdiv in {1, 2}: where the row-4 kernel runs (3D L = 96; FP32 at L = 192) it
+// takes P0 = 1536/L line pairs per workgroup or half of that; every other kernel
+// has one line-pair tile
 template <typename T, int L, int DIM>
 void launch_row_L(const RowArgs& a, long nsamples, hipStream_t st, int pdiv) {
-  constexpr int P0 = (1536 / L) < 1 ? 1 : 1536 / L;
-  constexpr int P1 = P0 / 2 >= 1 ? P0 / 2 : 1;
   if constexpr (DIM == 3 && sizeof(T) == 4 && L == 768) {
     // FP32 at L = 768: the radix-24 kernel, two line pairs per workgroup, whatever
     // pdiv says (one transform plan per precision and size keeps results
@@ -33,17 +33,18 @@ void launch_row_L(const RowArgs& a, long nsamples, hipStream_t st, int pdiv) {
     const long blocks = a.nlines / 2 * nsamples;
     hipLaunchKernelGGL((euler_row6_kernel<T>), dim3((unsigned)blocks), dim3(Row6Cfg::NT), 0, st, a);
     (void)pdiv;
-  } else if constexpr (DIM == 3 && L == 384) {
-    // 3D at L = 384: two line pairs per workgroup with a single twelve-sequence
-    // forward round, whatever pdiv says (one plan per size)
-    const long blocks = a.nlines / 4 * nsamples;
-    hipLaunchKernelGGL((euler_row6p_kernel<T>), dim3((unsigned)blocks), dim3(Row6pCfg::NT), 0, st, a);
-    (void)pdiv;
-  } else if constexpr (DIM == 3) {
+  } else if constexpr (DIM == 3 && (L == 96 || (L == 192 && sizeof(T) == 4))) {
+    constexpr int P0 = 1536 / L, P1 = P0 / 2;
     if (pdiv == 2)
       launch_row4_P<T, L, P1>(a, nsamples, st);
     else
       launch_row4_P<T, L, P0>(a, nsamples, st);
+  } else if constexpr (DIM == 3) {
+    // 3D at L = 384, and FP64 at L = 192: 768/L line pairs per workgroup with a
+    // single forward round, whatever pdiv says (one plan per precision and size)
+    const long blocks = a.nlines / (2 * Row6pCfg<L>::P) * nsamples;
+    hipLaunchKernelGGL((euler_row6p_kernel<T, L>), dim3((unsigned)blocks), dim3(Row6pCfg<L>::NT), 0, st, a);
+    (void)pdiv;
   } else if constexpr (L == 1536) {
     // 2D at L = 1536: one line pair per 192-thread workgroup, 24 values per thread
     const long blocks = a.nlines / 2 * nsamples;
@@ -66,7 +67,10 @@ template <typename T, int DIM>
 int launch_row_T(int L, const RowArgs& a, long nsamples, hipStream_t st, int pdiv) {
   switch (L) {
     case 96: launch_row_L<T, 96, DIM>(a, nsamples, st, pdiv); return 0;
-    case 192: launch_row_L<T, 192, DIM>(a, nsamples, st, pdiv); return 0;
+    case 192:
+      if (DIM == 3 && sizeof(T) == 8 && a.nlines % 8 != 0) return 2;  // four line pairs per workgroup
+      launch_row_L<T, 192, DIM>(a, nsamples, st, pdiv);
+      return 0;
     case 384:
       if (DIM == 3 && a.nlines % 4 != 0) return 2;  // two line pairs per workgroup
       launch_row_L<T, 384, DIM>(a, nsamples, st, pdiv);

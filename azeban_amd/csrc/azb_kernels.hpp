@@ -584,7 +584,6 @@ __device__ __forceinline__ void row_separate_store(const cx<T>* v, unsigned lane
 
 template <int L>
 struct Row4Plan;
-template <> struct Row4Plan<384> { using Inv = Plan<12, 2, 4, 4>; using Fwd = Plan<4, 4, 2, 12>; };
 template <> struct Row4Plan<192> { using Inv = Plan<12, 4, 4>; using Fwd = Plan<4, 4, 12>; };
 template <> struct Row4Plan<96> { using Inv = Plan<12, 2, 4>; using Fwd = Plan<4, 2, 12>; };
 
@@ -1302,45 +1301,63 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void e
 }
 
 // ---------------------------------------------------------------------------
-// 3D row kernel at L = 384 (N = 256) with one forward round: two line pairs per
-// 192-thread workgroup.  The inverse is the row-4 kernel's (six packed sequences,
-// 12 values per thread, plan 12.2.4.4) with its two middle passes in half-waves
-// (each sequence in its own half-wave: two of the three exchanges without a
-// workgroup barrier); all six products of a pair are formed at once and one
-// twelve-sequence forward transform (24 values per thread, plan 4.4.24, exchanges
-// in real/imaginary halves in FP64) replaces two rounds of six.  The radix-24 last
-// pass keeps butterfly j of a sequence in lane j of a 16-lane group, so the
+// 3D row kernel at L = 384 and 192 (N = 256, 128) with one forward round:
+// P = 768/L line pairs per 192-thread workgroup.  The inverse is the row-4
+// kernel's (3P packed sequences, 12 values per thread, plans 12.2.4.4 / 12.4.4;
+// at L = 384 its two middle passes run in half-waves, each sequence in its own
+// half-wave, so two of the three exchanges need no workgroup barrier); all six
+// products of a pair are formed at once and one 6P-sequence forward transform
+// (24 values per thread, plans 4.4.24 / 4.2.24, exchanges in real/imaginary
+// halves in FP64) replaces two rounds of 3P.  (At L = 96 the per-sequence pad
+// slots of 24 sequences would not leave room for four workgroups per CU; the
+// row-4 kernel stays there, and for FP32 at L = 192, where its four waves per
+// SIMD measured faster.)  The radix-24 last
+// pass keeps butterfly j of a sequence in lane j of an L/24-lane group, so the
 // separation of each packed spectrum stays within a wave.
 // ---------------------------------------------------------------------------
+template <int L>
+struct Row6pPlan;
+template <> struct Row6pPlan<384> { using Inv = Plan<12, 2, 4, 4>; using Fwd = Plan<4, 4, 24>; };
+template <> struct Row6pPlan<192> { using Inv = Plan<12, 4, 4>; using Fwd = Plan<4, 2, 24>; };
+
+template <int L_>
 struct Row6pCfg {
-  static constexpr int L = 384;
+  static constexpr int L = L_;
   static constexpr int NT = 192;
+  static constexpr int P = 768 / L;  // line pairs per workgroup
   static constexpr int NBI = L / 12, NBF = L / 24;
   // sequence strides chosen so that four workgroups' exchange buffers fit a CU
   static constexpr int SOFFI = 5, SOFFF = 5;
-  using PI = Plan<12, 2, 4, 4>;
-  using PF = Plan<4, 4, 24>;
+  // middle passes in half-waves where a half-wave holds one sequence (L = 384)
+  static constexpr unsigned MIDH = L == 384 ? 0b0110u : 0u;
+  // forward pad period (elements): one pad slot per 256 bytes at L = 384; at
+  // L = 192 every 16 elements, so that 24 sequences' pad slots fit the budget
+  template <typename T>
+  static constexpr int fwd_pade() { return L == 384 ? (sizeof(T) == 8 ? 32 : 0) : 16; }
+  using PI = typename Row6pPlan<L>::Inv;
+  using PF = typename Row6pPlan<L>::Fwd;
 };
 
-template <typename T>
+template <typename T, int L>
 __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3, 3))) void euler_row6p_kernel(RowArgs a) {
-  using C = Row6pCfg;
-  constexpr int L = C::L, NT = C::NT, NBI = C::NBI, NBF = C::NBF;
+  using C = Row6pCfg<L>;
+  constexpr int NT = C::NT, NBI = C::NBI, NBF = C::NBF, P = C::P;
   constexpr bool F64 = sizeof(T) == 8;
-  using FI = BlockFFT2<T, L, 6, NT, +1, false, 3, MAP_SEQ1, MAP_SEQ, true, C::SOFFI, 0u, 0, 0b0110u>;
-  using FF = BlockFFT2<T, L, 12, NT, -1, F64, 6, MAP_SEQ, MAP_SEQ1, true, C::SOFFF, 0u, F64 ? 32 : 0>;
+  using FI = BlockFFT2<T, L, 3 * P, NT, +1, false, 3, MAP_SEQ1, MAP_SEQ, true, C::SOFFI, 0u, 0, C::MIDH>;
+  using FF = BlockFFT2<T, L, 6 * P, NT, -1, F64, 6, MAP_SEQ, MAP_SEQ1, true, C::SOFFF, 0u,
+                       C::template fwd_pade<T>()>;
   static_assert(FI::E == 12 && FF::E == 24, "thread layout");
   constexpr int XWORDS = FI::lds_words > FF::lds_words ? FI::lds_words : FF::lds_words;
   static_assert(4 * (XWORDS * (int)sizeof(T) + 3 * 8) <= 163840, "four workgroups per CU");
   __shared__ T X[XWORDS];
   __shared__ double red[NT / 64];
-  const long bps = a.nlines / 4;  // workgroups per sample
+  const long bps = a.nlines / (2 * P);  // workgroups per sample
   const long sample = blockIdx.x / bps;
   DevState* st = a.st + sample;
   if (st->done) return;
 
   const int t = threadIdx.x;
-  const long line0 = (blockIdx.x % bps) * 4;
+  const long line0 = (blockIdx.x % bps) * (2 * P);
   const cx<T>* __restrict__ in = (const cx<T>*)a.in + sample * a.in_ss;
   cx<T>* __restrict__ out = (cx<T>*)a.out + sample * a.out_ss;
   const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
@@ -1366,7 +1383,7 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
   double lmax = (double)lm;
   asm volatile("" : "+v"(lmax));
 
-  // products of this thread's pair (t / 96) at its 4 positions: w[m*4 + r] =
+  // products of this thread's pair (t / (L/4)) at its 4 positions: w[m*4 + r] =
   // B_m (line a) + i B_m (line b), m = 0..5: B11, B21, B22, B31, B32, B33
   cx<T> w[24];
 #pragma unroll
