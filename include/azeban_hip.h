@@ -173,10 +173,11 @@ int azb_profile_reset(azb_solver* s);
 /* ---- autotuning of the kernel launch shapes --------------------------------
  * Per kernel family (AZB_PROF_*): the column passes take tiles of TS0, TS0/2 or
  * TS0/4 columns (tsdiv 1, 2, 4; TS0 = one 128-byte line), the row kernel P0 or
- * P0/2 line pairs per workgroup (pdiv 1, 2).  azb_autotune times every candidate
- * on the device with the given state (reps RHS evaluations each, output into the
- * solver's own buffers) and keeps the fastest per family.  Results do not depend
- * on the choice.  Synchronising. */
+ * P0/2 line pairs per workgroup (pdiv 1, 2) where it has two shapes (3D N = 64,
+ * and FP32 at 3D N = 128; every other row kernel has one).  azb_autotune times
+ * every candidate on the device with the given state (reps RHS evaluations each,
+ * output into the solver's own buffers) and keeps the fastest per family.
+ * Results do not depend on the choice.  Synchronising. */
 int azb_autotune(azb_solver* s, const void* u_hat, int reps);
 int azb_set_tuning(azb_solver* s, const int* tsdiv, int pdiv);
 int azb_get_tuning(const azb_solver* s, int* tsdiv, int* pdiv);
