@@ -1215,12 +1215,16 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void e
   using C = Row2pCfg;
   constexpr int L = C::L, NT = C::NT, NB = C::NB;
   constexpr bool F32 = sizeof(T) == 4;
-  using FI = BlockFFT2<T, L, 4, NT, +1, false, 4, MAP_STD, MAP_SEQ, true, C::SOFFI, 0u, 0, 0u, void, F32>;
+  // FP64: the inverse's exchanges go in real/imaginary halves too, so the buffer
+  // stays at the forward's 38.6 KB and four workgroups fit a CU
+  constexpr bool SPI = !F32;
+  using FI = BlockFFT2<T, L, 4, NT, +1, SPI, 4, MAP_STD, MAP_SEQ, true, C::SOFFI, 0u, 0, 0u, void, F32>;
   using FF = std::conditional_t<F32,
                                 BlockFFT2<T, L, 6, NT, -1, false, 6, MAP_SEQ, MAP_SEQ1, true, 5, 0u, 0, 0b010u, void, true>,
                                 BlockFFT2<T, L, 6, NT, -1, true, 6, MAP_SEQ, MAP_SEQ1, true, Row6Cfg::SOFFF, 0u, 32,
                                           0b010u, typename Row6Cfg::FwdLay>>;
   using LY = ExLayout<T, L, 4, true, C::SOFFI>;
+  static_assert(std::is_same_v<LY, typename FI::template XL<0>>, "first-pass layout");
   static_assert(FI::E == 16 && FF::E == 24, "thread layout");
   constexpr int XWORDS = FI::lds_words > FF::lds_words ? FI::lds_words : FF::lds_words;
   __shared__ T X[XWORDS];
@@ -1241,7 +1245,7 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void e
   const T nrm = T(a.norm);
 
   cx<T> v[24];
-  {
+  if constexpr (!SPI) {
     unsigned tt = t;
     asm volatile("" : "+v"(tt));
     if (tt < (unsigned)(4 * NB)) {
@@ -1256,9 +1260,49 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void e
         X[2 * idx + 1] = v[r].y;
       }
     }
+    __syncthreads();
+    FI::run_after_first(v, t, X, twi, typename C::PI{});
+  } else {
+    // the first pass's outputs go to LDS in two halves (real, then imaginary parts,
+    // one T per slot); pass 1 reads each half with its own mapping, as an exchange
+    // of the split transform does, then the transform continues from pass 1
+    constexpr int R1 = 8;
+    unsigned tt = t;
+    asm volatile("" : "+v"(tt));
+    const bool writer = tt < (unsigned)(4 * NB);
+    const unsigned s0 = tt / NB, j0 = tt % NB;
+    if (writer) {
+      const long l = line0 + s0;
+      row_first_pass<T, NB, 24>(v, in + l * a.ls, in + (a.nlines + l) * a.ls, j0);
+    }
+    cx<T> x[16];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+      if (c == 1) __syncthreads();  // every read of the real half is done
+      if (writer) {
+#pragma unroll
+        for (int r = 0; r < 24; ++r) X[LY::at(s0, 24 * j0 + r)] = c == 0 ? v[r].x : v[r].y;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int b = 0; b < 16 / R1; ++b) {
+        unsigned s1, j1;
+        bmap<MAP_STD, 4, NT, L, R1, 4>(tt, b, s1, j1);
+        const unsigned a0 = LY::at(s1, j1);
+#pragma unroll
+        for (int r = 0; r < R1; ++r) {
+          const unsigned idx = LY::linear(L / R1) ? a0 + r * LY::delta(L / R1) : LY::at(s1, j1 + r * (L / R1));
+          if (c == 0)
+            x[b * R1 + r].x = X[idx];
+          else
+            x[b * R1 + r].y = X[idx];
+        }
+      }
+    }
+    FI::template passes<1, 3, 24, 8, 4>(x, t, X, twi, Plan<8, 4>{});
+#pragma unroll
+    for (int e = 0; e < 16; ++e) v[e] = x[e];
   }
-  __syncthreads();
-  FI::run_after_first(v, t, X, twi, typename C::PI{});
 
   // v[g*4 + r]: u1 + i u2 of line g (0..3) at positions t + 192 r; products of pair
   // p (lines 2p, 2p + 1) into sequences 3p + m, m = B11, B12, B22
