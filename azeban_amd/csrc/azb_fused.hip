@@ -50,10 +50,10 @@ void launch_row_L(const RowArgs& a, long nsamples, hipStream_t st, int pdiv) {
     const long blocks = a.nlines / 2 * nsamples;
     hipLaunchKernelGGL((euler_row2w_kernel<T>), dim3((unsigned)blocks), dim3(Row2wCfg::NT), 0, st, a);
     (void)pdiv;
-  } else if constexpr (L == 768) {
-    // 2D at L = 768: two line pairs per 192-thread workgroup, the 3D forward
-    const long blocks = a.nlines / 4 * nsamples;
-    hipLaunchKernelGGL((euler_row2p_kernel<T>), dim3((unsigned)blocks), dim3(Row2pCfg::NT), 0, st, a);
+  } else if constexpr (L == 768 || L == 384) {
+    // 2D at L = 768, 384: 1536/L line pairs per 192-thread workgroup, the 3D forward
+    const long blocks = a.nlines / (2 * Row2pCfg<L>::P) * nsamples;
+    hipLaunchKernelGGL((euler_row2p_kernel<T, L>), dim3((unsigned)blocks), dim3(Row2pCfg<L>::NT), 0, st, a);
     (void)pdiv;
   } else {
     // one shape: 768/L line pairs, 192 threads
@@ -72,7 +72,7 @@ int launch_row_T(int L, const RowArgs& a, long nsamples, hipStream_t st, int pdi
       launch_row_L<T, 192, DIM>(a, nsamples, st, pdiv);
       return 0;
     case 384:
-      if (DIM == 3 && a.nlines % 4 != 0) return 2;  // two line pairs per workgroup
+      if (a.nlines % (DIM == 3 ? 4 : 8) != 0) return 2;  // two (3D) or four (2D) line pairs per workgroup
       launch_row_L<T, 384, DIM>(a, nsamples, st, pdiv);
       return 0;
     case 768:

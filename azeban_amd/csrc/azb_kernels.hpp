@@ -972,7 +972,7 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 }
 
 // ---------------------------------------------------------------------------
-// 2D row kernel (L <= 384), the 3D kernel's layout with one product round.
+// 2D row kernel (L <= 192), the 3D kernel's layout with one product round.
 // Per workgroup: P = 768/L line pairs, 192 threads.  Inverse: 2P packed sequences
 // (u1 + i u2 of each line, plan 12 ...): the radix-12 first pass (loaded from HBM)
 // runs on the first 2P * L/12 = 128 threads, one butterfly each, the later passes
@@ -984,13 +984,12 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3, 3))) voi
 // ---------------------------------------------------------------------------
 template <int L>
 struct Row2Plan;
-template <> struct Row2Plan<384> { using Inv = Plan<12, 8, 4>; using Fwd = Plan<4, 4, 2, 12>; };
 template <> struct Row2Plan<192> { using Inv = Plan<12, 4, 4>; using Fwd = Plan<4, 4, 12>; };
 template <> struct Row2Plan<96> { using Inv = Plan<12, 2, 4>; using Fwd = Plan<4, 2, 12>; };
 
 template <typename T, int L>
 struct Row2Cfg {
-  static_assert(L <= 384, "L = 768 and 1536 have kernels of their own (euler_row2p_kernel, euler_row2w_kernel)");
+  static_assert(L <= 192, "L = 384, 768 and 1536 have kernels of their own (euler_row2p_kernel, euler_row2w_kernel)");
   static constexpr int P = 768 / L;
   static constexpr int NT = P * L / 4;
   static constexpr int TSI = 2 * P, TSF = 3 * P;
@@ -1190,65 +1189,73 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void e
 }
 
 // ---------------------------------------------------------------------------
-// 2D row kernel at L = 768 (N = 512) with two line pairs per 192-thread workgroup
-// and the 3D kernels' forward: six sequences (B11, B12, B22 of each pair, packed
-// line a + i line b), 24 values per thread, plan 4.8.24 with the radix-8 pass in
-// half-waves; FP64 as euler_row6_kernel (exchanges in real/imaginary halves,
-// swizzled second exchange), FP32 as euler_row8_kernel (per-pass twiddle tables).
-// Inverse (plan 24.8.4): the radix-24 first pass from HBM on 128 threads (four
-// sequences u1 + i u2 of lines 0..3, butterfly j in lane j of a half-wave), written
-// to LDS, then 16 values per thread on all 192 threads; the radix-4 last pass
-// leaves all four sequences at positions t + 192 r in thread t, where the
+// 2D row kernel at L = 768 and 384 (N = 512, 256) with P = 1536/L line pairs per
+// 192-thread workgroup and the 3D kernels' forward: 3P sequences (B11, B12, B22 of
+// each pair, packed line a + i line b), 24 values per thread, plans 4.8.24 / 4.4.24
+// (at L = 768 the radix-8 pass in half-waves; FP64 as euler_row6_kernel, exchanges
+// in real/imaginary halves and swizzled second exchange; FP32 as
+// euler_row8_kernel, per-pass twiddle tables).  Inverse (plans 24.8.4 / 24.4.4):
+// the radix-24 first pass from HBM on 128 threads (2P sequences u1 + i u2 of lines
+// 0..2P-1, butterfly j in lane j of an L/24-lane group), written to LDS, then 16
+// values per thread on all 192 threads; the radix-4 last pass leaves four
+// sequences (two pairs) at positions t % (L/4) + (L/4) r in thread t, where the
 // products are formed.  Two exchanges per transform instead of three.
 // ---------------------------------------------------------------------------
+template <int L_>
 struct Row2pCfg {
-  static constexpr int L = 768;
+  static constexpr int L = L_;
   static constexpr int NT = 192;
-  static constexpr int NB = L / 24;  // radix-24 butterflies per sequence
+  static constexpr int P = 1536 / L;  // line pairs per workgroup
+  static constexpr int NB = L / 24;   // radix-24 butterflies per sequence
   static constexpr int SOFFI = 11;
-  using PI = Plan<24, 8, 4>;
-  using PF = Plan<4, 8, 24>;
+  static constexpr int R1 = L == 768 ? 8 : 4;  // second inverse radix
+  using PI = Plan<24, R1, 4>;
+  using PF = Plan<4, R1, 24>;
 };
 
-template <typename T>
+template <typename T, int L>
 __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void euler_row2p_kernel(RowArgs a) {
-  using C = Row2pCfg;
-  constexpr int L = C::L, NT = C::NT, NB = C::NB;
+  using C = Row2pCfg<L>;
+  constexpr int NT = C::NT, NB = C::NB, P = C::P, R1 = C::R1;
   constexpr bool F32 = sizeof(T) == 4;
   // FP64: the inverse's exchanges go in real/imaginary halves too, so the buffer
-  // stays at the forward's 38.6 KB and four workgroups fit a CU
+  // stays at the forward's size and four workgroups fit a CU
   constexpr bool SPI = !F32;
-  using FI = BlockFFT2<T, L, 4, NT, +1, SPI, 4, MAP_STD, MAP_SEQ, true, C::SOFFI, 0u, 0, 0u, void, F32>;
-  using FF = std::conditional_t<F32,
-                                BlockFFT2<T, L, 6, NT, -1, false, 6, MAP_SEQ, MAP_SEQ1, true, 5, 0u, 0, 0b010u, void, true>,
-                                BlockFFT2<T, L, 6, NT, -1, true, 6, MAP_SEQ, MAP_SEQ1, true, Row6Cfg::SOFFF, 0u, 32,
-                                          0b010u, typename Row6Cfg::FwdLay>>;
-  using LY = ExLayout<T, L, 4, true, C::SOFFI>;
+  // per-pass twiddle tables: FP32 at L = 768 (the 3D FP32 row kernel's, same plans)
+  constexpr bool PT = F32 && L == 768;
+  using FI = BlockFFT2<T, L, 2 * P, NT, +1, SPI, 4, MAP_STD, MAP_SEQ, true, C::SOFFI, 0u, 0, 0u, void, PT>;
+  using FF768 = std::conditional_t<
+      F32, BlockFFT2<T, L, 6, NT, -1, false, 6, MAP_SEQ, MAP_SEQ1, true, 5, 0u, 0, 0b010u, void, true>,
+      BlockFFT2<T, L, 6, NT, -1, true, 6, MAP_SEQ, MAP_SEQ1, true, Row6Cfg::SOFFF, 0u, 32, 0b010u,
+                typename Row6Cfg::FwdLay>>;
+  using FF384 = BlockFFT2<T, L, 3 * P, NT, -1, !F32, 6, MAP_SEQ, MAP_SEQ1, true, 5, 0u, F32 ? 0 : 32>;
+  using FF = std::conditional_t<L == 768, FF768, FF384>;
+  using LY = ExLayout<T, L, 2 * P, true, C::SOFFI>;
   static_assert(std::is_same_v<LY, typename FI::template XL<0>>, "first-pass layout");
   static_assert(FI::E == 16 && FF::E == 24, "thread layout");
   constexpr int XWORDS = FI::lds_words > FF::lds_words ? FI::lds_words : FF::lds_words;
+  static_assert(L == 768 || 4 * (XWORDS * (int)sizeof(T) + 3 * 8) <= 163840, "four workgroups per CU");
   __shared__ T X[XWORDS];
   __shared__ double red[NT / 64];
-  const long bps = a.nlines / 4;  // workgroups per sample
+  const long bps = a.nlines / (2 * P);  // workgroups per sample
   const long sample = blockIdx.x / bps;
   DevState* st = a.st + sample;
   if (st->done) return;
 
   const int t = threadIdx.x;
-  const long line0 = (blockIdx.x % bps) * 4;
+  const long line0 = (blockIdx.x % bps) * (2 * P);
   const cx<T>* __restrict__ in = (const cx<T>*)a.in + sample * a.in_ss;
   cx<T>* __restrict__ out = (cx<T>*)a.out + sample * a.out_ss;
   const cx<T>* __restrict__ tw = (const cx<T>*)a.tw;
-  // FP32: per-pass twiddle tables (the 3D FP32 row kernel's, same plans)
-  const cx<T>* twi = F32 ? tw + Row8Cfg<float, 768>::TW_INV : tw;
-  const cx<T>* twf = F32 ? tw + Row8Cfg<float, 768>::TW_FWD : tw;
+  const cx<T>* twi = PT ? tw + Row8Cfg<float, 768>::TW_INV : tw;
+  const cx<T>* twf = PT ? tw + Row8Cfg<float, 768>::TW_FWD : tw;
   const T nrm = T(a.norm);
 
   cx<T> v[24];
   if constexpr (!SPI) {
     unsigned tt = t;
     asm volatile("" : "+v"(tt));
-    if (tt < (unsigned)(4 * NB)) {
+    if (tt < (unsigned)(2 * P * NB)) {
       const unsigned s = tt / NB, j = tt % NB;  // sequence s: u1 + i u2 of line line0 + s
       const long l = line0 + s;
       row_first_pass<T, NB, 24>(v, in + l * a.ls, in + (a.nlines + l) * a.ls, j);
@@ -1266,10 +1273,9 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void e
     // the first pass's outputs go to LDS in two halves (real, then imaginary parts,
     // one T per slot); pass 1 reads each half with its own mapping, as an exchange
     // of the split transform does, then the transform continues from pass 1
-    constexpr int R1 = 8;
     unsigned tt = t;
     asm volatile("" : "+v"(tt));
-    const bool writer = tt < (unsigned)(4 * NB);
+    const bool writer = tt < (unsigned)(2 * P * NB);
     const unsigned s0 = tt / NB, j0 = tt % NB;
     if (writer) {
       const long l = line0 + s0;
@@ -1287,7 +1293,7 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void e
 #pragma unroll
       for (int b = 0; b < 16 / R1; ++b) {
         unsigned s1, j1;
-        bmap<MAP_STD, 4, NT, L, R1, 4>(tt, b, s1, j1);
+        bmap<MAP_STD, 2 * P, NT, L, R1, 4>(tt, b, s1, j1);
         const unsigned a0 = LY::at(s1, j1);
 #pragma unroll
         for (int r = 0; r < R1; ++r) {
@@ -1299,13 +1305,14 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void e
         }
       }
     }
-    FI::template passes<1, 3, 24, 8, 4>(x, t, X, twi, Plan<8, 4>{});
+    FI::template passes<1, 3, 24, R1, 4>(x, t, X, twi, Plan<R1, 4>{});
 #pragma unroll
     for (int e = 0; e < 16; ++e) v[e] = x[e];
   }
 
-  // v[g*4 + r]: u1 + i u2 of line g (0..3) at positions t + 192 r; products of pair
-  // p (lines 2p, 2p + 1) into sequences 3p + m, m = B11, B12, B22
+  // v[g*4 + r]: u1 + i u2 of line 4 (t / (L/4)) + g (g = 0..3) at positions
+  // t % (L/4) + (L/4) r; products of the thread's pair p (lines 2p, 2p + 1 of the
+  // four) into its sequences 3p + m, m = B11, B12, B22
   T lm = T(0);
   cx<T> w[24];
 #pragma unroll
