@@ -1214,21 +1214,21 @@ struct Row2pCfg {
 };
 
 template <typename T, int L>
-__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void euler_row2p_kernel(RowArgs a) {
+__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 3 : 4))) void euler_row2p_kernel(RowArgs a) {
   using C = Row2pCfg<L>;
   constexpr int NT = C::NT, NB = C::NB, P = C::P, R1 = C::R1;
   constexpr bool F32 = sizeof(T) == 4;
-  // FP64: the inverse's exchanges go in real/imaginary halves too, so the buffer
-  // stays at the forward's size and four workgroups fit a CU
-  constexpr bool SPI = !F32;
+  // every exchange in real/imaginary halves: FP64 keeps four workgroups per CU,
+  // FP32 (19 KB) five at four waves per SIMD
+  constexpr bool SPI = true;
   // per-pass twiddle tables: FP32 at L = 768 (the 3D FP32 row kernel's, same plans)
   constexpr bool PT = F32 && L == 768;
   using FI = BlockFFT2<T, L, 2 * P, NT, +1, SPI, 4, MAP_STD, MAP_SEQ, true, C::SOFFI, 0u, 0, 0u, void, PT>;
   using FF768 = std::conditional_t<
-      F32, BlockFFT2<T, L, 6, NT, -1, false, 6, MAP_SEQ, MAP_SEQ1, true, 5, 0u, 0, 0b010u, void, true>,
+      F32, BlockFFT2<T, L, 6, NT, -1, true, 6, MAP_SEQ, MAP_SEQ1, true, 5, 0u, 0, 0b010u, void, true>,
       BlockFFT2<T, L, 6, NT, -1, true, 6, MAP_SEQ, MAP_SEQ1, true, Row6Cfg::SOFFF, 0u, 32, 0b010u,
                 typename Row6Cfg::FwdLay>>;
-  using FF384 = BlockFFT2<T, L, 3 * P, NT, -1, !F32, 6, MAP_SEQ, MAP_SEQ1, true, 5, 0u, F32 ? 0 : 32>;
+  using FF384 = BlockFFT2<T, L, 3 * P, NT, -1, true, 6, MAP_SEQ, MAP_SEQ1, true, 5, 0u, F32 ? 0 : 32>;
   using FF = std::conditional_t<L == 768, FF768, FF384>;
   using LY = ExLayout<T, L, 2 * P, true, C::SOFFI>;
   static_assert(std::is_same_v<LY, typename FI::template XL<0>>, "first-pass layout");
