@@ -1390,12 +1390,14 @@ struct Row6pCfg {
 };
 
 template <typename T, int L>
-__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3, 3))) void euler_row6p_kernel(RowArgs a) {
+__global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? 3 : 4))) void euler_row6p_kernel(RowArgs a) {
   using C = Row6pCfg<L>;
   constexpr int NT = C::NT, NBI = C::NBI, NBF = C::NBF, P = C::P;
   constexpr bool F64 = sizeof(T) == 8;
   using FI = BlockFFT2<T, L, 3 * P, NT, +1, false, 3, MAP_SEQ1, MAP_SEQ, true, C::SOFFI, 0u, 0, C::MIDH>;
-  using FF = BlockFFT2<T, L, 6 * P, NT, -1, F64, 6, MAP_SEQ, MAP_SEQ1, true, C::SOFFF, 0u,
+  // forward exchanges in real/imaginary halves: FP64 to fit four workgroups per
+  // CU, FP32 (20 KB) to run four waves per SIMD
+  using FF = BlockFFT2<T, L, 6 * P, NT, -1, true, 6, MAP_SEQ, MAP_SEQ1, true, C::SOFFF, 0u,
                        C::template fwd_pade<T>()>;
   static_assert(FI::E == 12 && FF::E == 24, "thread layout");
   constexpr int XWORDS = FI::lds_words > FF::lds_words ? FI::lds_words : FF::lds_words;
