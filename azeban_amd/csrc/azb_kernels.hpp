@@ -1192,9 +1192,9 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(3))) void e
 // 2D row kernel at L = 768 and 384 (N = 512, 256) with P = 1536/L line pairs per
 // 192-thread workgroup and the 3D kernels' forward: 3P sequences (B11, B12, B22 of
 // each pair, packed line a + i line b), 24 values per thread, plans 4.8.24 / 4.4.24
-// (at L = 768 the radix-8 pass in half-waves; FP64 as euler_row6_kernel, exchanges
-// in real/imaginary halves and swizzled second exchange; FP32 as
-// euler_row8_kernel, per-pass twiddle tables).  Inverse (plans 24.8.4 / 24.4.4):
+// (at L = 768 the radix-8 pass in half-waves; FP64 as euler_row6_kernel, swizzled
+// second exchange; FP32 as euler_row8_kernel, per-pass twiddle tables; every
+// exchange in real/imaginary halves).  Inverse (plans 24.8.4 / 24.4.4):
 // the radix-24 first pass from HBM on 128 threads (2P sequences u1 + i u2 of lines
 // 0..2P-1, butterfly j in lane j of an L/24-lane group), written to LDS, then 16
 // values per thread on all 192 threads; the radix-4 last pass leaves four
@@ -1359,7 +1359,7 @@ __global__ __launch_bounds__(192) __attribute__((amdgpu_waves_per_eu(sizeof(T) =
 // half-wave, so two of the three exchanges need no workgroup barrier); all six
 // products of a pair are formed at once and one 6P-sequence forward transform
 // (24 values per thread, plans 4.4.24 / 4.2.24, exchanges in real/imaginary
-// halves in FP64) replaces two rounds of 3P.  (At L = 96 the per-sequence pad
+// halves) replaces two rounds of 3P.  (At L = 96 the per-sequence pad
 // slots of 24 sequences would not leave room for four workgroups per CU; the
 // row-4 kernel stays there, and for FP32 at L = 192, where its four waves per
 // SIMD measured faster.)  The radix-24 last
